@@ -1,0 +1,222 @@
+/*
+ * mdrl.h -- C ABI of libmdrl, the MI355X (gfx950) range-limited force path.
+ *
+ * Plain C: no C++ types, no exceptions and no device types cross this boundary.
+ * Every entry point returns an int status (MDRL_OK = 0, negative MDRL_E* on error;
+ * md_rl_last_error() gives the message of the calling thread's last failure), except
+ * MD_Evaluation_Model(), which keeps the reference's "return 1" convention.
+ *
+ * Ownership: the caller owns every host array it passes; the library copies what it
+ * needs and owns all device memory inside opaque handles.  A handle is not
+ * thread-safe; distinct handles may be used from distinct threads.  Calls are
+ * synchronous unless the name ends in _async; *_async calls take a hipStream_t
+ * passed as void* (NULL = the handle's own stream).
+ *
+ * Units on the system path: positions in Angstrom, velocities in Angstrom/fs, forces
+ * in kJ/mol/Angstrom, masses in amu, energies in kJ/mol, time step in fs, charges in e.
+ *
+ * Reference paths: D/ = Range_Limited_demo/SourceCode/, R/ = Range_Limited/,
+ * O/ = Range_Limited_demo/SourceCode/OpenCL_HDL_Lib/, M/ = D/MatlabScripts/.
+ */
+#ifndef MDRL_H
+#define MDRL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDRL_VERSION 1
+
+/* ---- status ---------------------------------------------------------------- */
+enum {
+    MDRL_OK = 0,
+    MDRL_EINVAL = -1,      /* bad argument */
+    MDRL_ENOMEM = -2,      /* host or device allocation failed */
+    MDRL_EIO = -3,         /* file missing or malformed */
+    MDRL_EHIP = -4,        /* HIP runtime error */
+    MDRL_ECELL = -5,       /* cell geometry: cell edge < rc, or a cell over capacity */
+    MDRL_ENONFINITE = -6,  /* NaN/Inf position, velocity or force */
+    MDRL_ESATURATED = -7,  /* a fixed-point force contribution was clamped */
+    MDRL_ENODEV = -8,      /* no gfx950 device visible */
+};
+
+/* Message for the calling thread's most recent failing call ("" if none). */
+const char *md_rl_last_error(void);
+int md_rl_version(void);
+
+/* ---- interpolation tables ---------------------------------------------------- */
+/* First-order tables over r^2: entry index = {exp8(r2) - exp_offset, mantissa[22:15]}
+ * (256 bins per octave), term(r2) = c1*r2 + c0.  Replaces the lut*_* ROMs initialised
+ * from c{0,1}_{14,8,3}.mif/.hex (D/lut0_14.v:34-38, D/RL_LJ_Evaluate_Pairs_1st_Order.v:
+ * 176-199) and the MATLAB writers (M/LJ_Coulomb_no_smooth_poly_interpolation_function.m).
+ * ApoA1 layout: exp_offset 121, 14 segments; LJArgon layout: 125, 9 segments. */
+enum {
+    MDRL_TERM_14 = 0, MDRL_TERM_8 = 1, MDRL_TERM_3 = 2,   /* force terms */
+    MDRL_TERM_12 = 3, MDRL_TERM_6 = 4, MDRL_TERM_1 = 5,   /* energy terms */
+    MDRL_NTERMS = 6
+};
+typedef struct md_rl_table md_rl_table;
+
+/* Reads c0_<k>.<ext> / c1_<k>.<ext> for k in 14,8,3,12,6,1 from dir; ext is tried in
+ * the order mif, hex, txt.  The 14 and 8 terms are required, the others optional. */
+int md_rl_table_load(md_rl_table **out, const char *dir, int exp_offset, int nseg);
+/* Builds a table from caller arrays (nseg*256 entries each; NULL = term absent). */
+int md_rl_table_from_arrays(md_rl_table **out, int exp_offset, int nseg,
+                            const float *const c0[MDRL_NTERMS],
+                            const float *const c1[MDRL_NTERMS]);
+/* Reads one table file (Quartus .mif, Intel .hex with 4-byte records, or MATLAB .txt
+ * with one decimal per line) into out[cap]; returns the entry count or < 0. */
+int md_rl_table_read_file(const char *path, float *out, int cap);
+int md_rl_table_get(const md_rl_table *t, int term, const float **c0, const float **c1,
+                    int *nentries, int *exp_offset, int *nseg);
+void md_rl_table_free(md_rl_table *t);
+/* Directory of the ApoA1-layout tables shipped with the library. */
+const char *md_rl_default_table_dir(void);
+/* Loads the shipped ApoA1 tables (offset 121, 14 segments). */
+int md_rl_table_load_default(md_rl_table **out);
+
+/* ---- pair batch ---------------------------------------------------------------- */
+/* Evaluates independent (reference, neighbour) pairs.  Replaces the OpenCL kernel
+ * LJ(ref_id, neighbor_id, ref_x..z, neighbor_x..z, Force_out_x..z)
+ * (O/test_hdl_lib/device/LJ.cl:31-80) and its RTL function
+ * float4 RL_LJ_Evaluation(int2 particle_id, float4 reference_pos, float4 neighbor_pos)
+ * (O/RL_LJ_Evaluation.h:1, O/HDL_Source/RL_LJ_Evaluation_OpenCL_Top.v:94-134):
+ * d = ref - nbr, r2 = |d|^2, F = t14 - t8 (unit-coefficient tables), out = F*d, and
+ * 0 for pairs with r2 == 0 or r2 > rc2 (O/HDL_Source/RL_LJ_Evaluate_Pairs_1st_Order.v:
+ * 150-152).  ref_id/nbr_id are carried as in the reference and may be NULL.
+ * Host pointers; synchronous. */
+enum {
+    MDRL_PAIR_PARITY = 1,   /* unfused r2 and c1*r2+c0, as the MATLAB golden model */
+    MDRL_PAIR_STRICT = 2,   /* accept r2 < rc2 (MATLAB) instead of r2 <= rc2 (HDL) */
+    MDRL_PAIR_COULOMB = 4,  /* add p_qq * t3 (RL_Evaluate_Pairs_1st_Order.v) */
+};
+int md_rl_eval_pairs(const md_rl_table *t, const int32_t *ref_id, const int32_t *nbr_id,
+                     const float *ref_x, const float *ref_y, const float *ref_z,
+                     const float *nbr_x, const float *nbr_y, const float *nbr_z,
+                     float *force_x, float *force_y, float *force_z, int64_t npairs,
+                     float rc2, int flags);
+/* As md_rl_eval_pairs with the per-pair multipliers of D/RL_Evaluate_Pairs_1st_Order.v:
+ * 53-55 (p_a, p_b, p_qq; NULL = 1, 1, 0): F = p_a*t14 - p_b*t8 + p_qq*t3.  force_s
+ * (may be NULL) receives the scalar F. */
+int md_rl_eval_pairs_ex(const md_rl_table *t, const float *ref_xyz, const float *nbr_xyz,
+                        const float *p_a, const float *p_b, const float *p_qq,
+                        float *force_s, float *force_xyz, int64_t npairs, float rc2,
+                        int flags);
+
+/* ---- system path ------------------------------------------------------------------ */
+/* Half-shell cell list (D/Particle_Pair_Gen_HalfShell_64_Cells.v), r2 filter
+ * (D/Filter_Logic.v:234), table force (D/RL_LJ_Evaluate_Pairs_1st_Order.v), Newton's-
+ * third-law accumulation (D/Partial_Force_Acc.v, D/Force_Write_Back_Controller.v) and
+ * Motion_Update (D/Motion_Update.v), on the C model's particle layout
+ * float particle[N][11] (R/C_Model_ForceEval/MD_Evaluation_Model.cpp:35):
+ * 0-2 position, 3-5 velocity, 6-8 force, 9 potential energy, 10 kinetic energy. */
+enum {
+    MDRL_ACC_FP32 = 1,       /* FP32 atomics instead of int64 fixed point (not deterministic) */
+    MDRL_EVAL_ANALYTIC = 2,  /* FP32 direct r^-14/r^-8/r^-3 instead of the tables */
+    MDRL_EVAL_PARITY = 4,    /* unfused arithmetic, d = (xi - xj) - image (reference order) */
+    MDRL_KICK_DRIFT = 8,     /* reference integrator: v += a*dt; x += v*dt (no half kicks) */
+    MDRL_NO_COULOMB = 16,    /* drop the r^-3 term even if charges are set */
+};
+
+typedef struct md_rl_params {
+    int n;              /* atoms */
+    float box[3];       /* periodic box edges (A) */
+    float rc;           /* cutoff (A) */
+    int ncell[3];       /* cells per axis; 0 = floor(box/rc). Cell edge must be >= rc */
+    float dt;           /* time step (fs) */
+    int ntypes;         /* atom types (1..64) */
+    const float *eps;   /* ntypes: LJ epsilon (kJ/mol), Lorentz-Berthelot mixing */
+    const float *sigma; /* ntypes: LJ sigma (A) */
+    const float *mass;  /* ntypes: mass (amu) */
+    float coulomb_k;    /* Coulomb constant (kJ A / mol / e^2); 0 = no r^-3 term */
+    int flags;          /* MDRL_ACC_* | MDRL_EVAL_* | MDRL_KICK_DRIFT | ... */
+    const md_rl_table *table;  /* NULL = the shipped ApoA1 tables */
+    int cell_capacity;  /* max atoms per cell; 0 = automatic */
+    int reserved[7];
+} md_rl_params;
+
+typedef struct md_rl_ctx md_rl_ctx;
+
+typedef struct md_rl_stats {
+    int64_t pair_evals;       /* in-cutoff unordered pairs of the last force evaluation */
+    int64_t candidates;       /* half-shell candidate pairs tested */
+    int max_cell_count;       /* largest cell at the last rebin */
+    int flags;                /* sticky MDRL_F_* bits raised by the kernels */
+    double force_ms;          /* mean device time of the force kernel over the timed steps
+                                 of the last synchronised batch (HIP events on its stream) */
+    double step_ms;           /* mean device time of a full step over the same steps */
+    int timed_steps;          /* steps behind the two means */
+} md_rl_stats;
+enum { MDRL_F_SATURATED = 1, MDRL_F_NONFINITE = 2, MDRL_F_CELL_OVERFLOW = 4 };
+
+int md_rl_create(md_rl_ctx **out, const md_rl_params *p);
+/* Positions (cols 0-2) and velocities (cols 3-5) of n particles in the N x 11 layout;
+ * positions are wrapped into the box. */
+int md_rl_set_particles(md_rl_ctx *c, const float *particle, int n);
+/* Type index (0..ntypes-1) and charge (e) per particle; charge may be NULL. */
+int md_rl_set_types(md_rl_ctx *c, const int32_t *type, const float *charge, int n);
+/* Forces at the current positions (cols 6-8 after md_rl_get_particles). */
+int md_rl_force_eval(md_rl_ctx *c);
+/* nsteps velocity-Verlet (or kick-drift) steps: force, Motion_Update, rebin. */
+int md_rl_step(md_rl_ctx *c, int nsteps);
+int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream);
+int md_rl_sync(md_rl_ctx *c);
+/* Copies back positions, velocities (full step), forces and per-atom kinetic energy. */
+int md_rl_get_particles(md_rl_ctx *c, float *particle, int n);
+/* Raw int64 fixed-point forces (2^-32 kJ/mol/nm units), n x 3, input order. */
+int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n);
+/* FP64 energy of the current positions from the c*_{12,6,1} tables (potential),
+ * kinetic energy of the last full-step velocities and the pair virial. */
+int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9]);
+int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *s);
+/* Enables HIP-event timing of each force kernel and full step (off by default). */
+int md_rl_set_timing(md_rl_ctx *c, int on);
+int md_rl_destroy(md_rl_ctx *c);
+
+/* ---- the C model entry point ------------------------------------------------------- */
+/* Drop-in for int MD_Evaluation_Model() (R/C_Model_ForceEval/MD_Evaluation_Model.h:1):
+ * reads the PDB named by $MDRL_INPUT (default: the reference's ar_gas.pdb path), runs
+ * the reference's 100 kick-then-drift iterations of all-pairs LJ on the GPU with the
+ * reference's arithmetic, prints one line per iteration in the reference's format
+ * (:159) and returns 1; 0 on error. */
+int MD_Evaluation_Model(void);
+
+typedef struct md_rl_cmodel_cfg {
+    int n;                 /* particles (reference: 500) */
+    int iterations;        /* reference: 100 */
+    float eps, sigma, mass, dt, rc;   /* reference literals, :14-24 */
+    float box[3];          /* :26-28 */
+    int print;             /* print the per-iteration line */
+} md_rl_cmodel_cfg;
+/* Fills the reference's constants with its promotions (:11-29). */
+void md_rl_cmodel_default(md_rl_cmodel_cfg *cfg);
+/* Runs the C model on the GPU over particle[n][11] in place (positions and velocities
+ * in, everything out).  energies (may be NULL) receives 2 floats per iteration: the
+ * printed system energy and kinetic energy. */
+int md_rl_cmodel_run(const md_rl_cmodel_cfg *cfg, float *particle, float *energies);
+
+/* ---- synthetic systems ----------------------------------------------------------------- */
+/* Seeded generators of the benchmark systems (SplitMix64-seeded xoshiro256**), shared by
+ * the CPU baselines and the GPU so both see identical inputs. */
+typedef struct md_rl_system {
+    int n, ntypes;
+    float box[3];
+    float rc;
+    float dt;
+    float coulomb_k;
+    float *particle;   /* n x 11 (pos, vel; other columns 0) */
+    int32_t *type;     /* n */
+    float *charge;     /* n */
+    float eps[8], sigma[8], mass[8];
+} md_rl_system;
+/* which: 1 = 512-atom argon; 2 = 92,224-atom LJ+Coulomb; 3 = 1,048,576-atom;
+ * 4 = one replica of the batched set (index = replica, 0..511). */
+int md_rl_make_system(md_rl_system *s, int which, int index, uint64_t seed);
+void md_rl_free_system(md_rl_system *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

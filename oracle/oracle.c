@@ -1,0 +1,746 @@
+/*
+ * oracle.c -- CPU restatement of the reference's range-limited force path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h for the rules and for what pins each
+ * function).  Built with -O2 -ffp-contract=off and no -ffast-math so that every
+ * float operation below rounds exactly where it is written; fused operations are
+ * written as explicit fmaf() calls.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* tables                                                                     */
+/* ------------------------------------------------------------------------- */
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* D/RL_LJ_Evaluate_Pairs_1st_Order.v:176-199: rdaddr = {segment_id, bin_id},
+ * bin_id = r2[22:15], segment_id = r2[30:23] - offset when that lies in
+ * [0, SEGMENT_NUM), else 0 (the subtraction is unsigned in the HDL, so a
+ * negative difference is also out of range). */
+int or_lut_index(float r2, int exp_offset, int nseg)
+{
+    uint32_t u = f2u(r2);
+    int e = (int)((u >> 23) & 0xffu);
+    int seg = e - exp_offset;
+    if (seg < 0 || seg >= nseg) seg = 0;
+    int bin = (int)((u >> 15) & 0xffu);
+    return seg * 256 + bin;
+}
+
+/* M/Simulation_Verification_Raw_ApoA1_LJ_no_smooth.m:120-198. */
+void or_pair_golden(const float *c0_14, const float *c1_14, const float *c0_8,
+                    const float *c1_8, int exp_offset, int nseg, float rc2, int strict,
+                    const float *ref, const float *nbr, int64_t n, float *out)
+{
+    for (int64_t p = 0; p < n; ++p) {
+        float dx = ref[3 * p + 0] - nbr[3 * p + 0];             /* :132-134 */
+        float dy = ref[3 * p + 1] - nbr[3 * p + 1];
+        float dz = ref[3 * p + 2] - nbr[3 * p + 2];
+        float r2 = dx * dx + dy * dy;                           /* :135, left to right */
+        r2 = r2 + dz * dz;
+        int in = strict ? (r2 < rc2 && r2 > 0.0f) : (r2 <= rc2 && r2 > 0.0f);
+        float F = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (in) {
+            int idx = or_lut_index(r2, exp_offset, nseg);       /* :139-153 */
+            float t14 = c1_14[idx] * r2;                        /* :174 polyval, unfused */
+            t14 = t14 + c0_14[idx];
+            float t8 = c1_8[idx] * r2;                          /* :175 */
+            t8 = t8 + c0_8[idx];
+            F = t14 - t8;                                       /* :184 */
+            fx = F * dx;                                        /* :185-187 */
+            fy = F * dy;
+            fz = F * dz;
+        }
+        float *o = out + 8 * p;
+        o[0] = F; o[1] = fx; o[2] = fy; o[3] = fz;
+        o[4] = r2; o[5] = dx; o[6] = dy; o[7] = dz;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* C model                                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* R/C_Model_ForceEval/MD_Evaluation_Model.cpp:11-29, with the same promotions:
+ * double literals rounded to float on assignment, float/float divisions. */
+void or_cmodel_default_params(or_cmodel_params *p)
+{
+    float Nav = (float)6.022e23;                       /* :15 */
+    float Ar_weight = (float)39.95;                    /* :16 */
+    p->eps = (float)(1.995996 * 1.995996);             /* :17 */
+    p->sigma = (float)(0.1675 * 2);                    /* :18 */
+    p->mass = Ar_weight / Nav / 1000.0f;               /* :21 */
+    p->dt = (float)2E-15;                              /* :22 */
+    p->rc = 8.0f;                                      /* :23 */
+    p->box[0] = p->box[1] = p->box[2] = 24.0f;         /* :26-28 */
+    p->iterations = 100;                               /* :11 */
+}
+
+/* One PBC component, :81-98: dx -= L * (int)(dx/L +- 0.5), the +-0.5 in double. */
+static inline float cm_pbc(float d, float L)
+{
+    if (d >= 0) {
+        float q = d / L;
+        int k = (int)((double)q + 0.5);
+        d -= L * (float)k;
+    } else {
+        float q = d / L;
+        int k = (int)((double)q - 0.5);
+        d -= L * (float)k;
+    }
+    return d;
+}
+
+typedef struct { float s12, s6, c4, c48, c24, rc2; } cm_consts;
+
+static void cm_setup(const or_cmodel_params *p, cm_consts *c)
+{
+    c->s12 = (float)pow((double)p->sigma, 12.0);       /* :19 */
+    c->s6 = (float)pow((double)p->sigma, 6.0);         /* :20 */
+    c->c4 = 4.0f * p->eps;
+    c->c48 = 48.0f * p->eps;
+    c->c24 = 24.0f * p->eps;
+    c->rc2 = p->rc * p->rc;                            /* :24 */
+}
+
+/* The :72-124 neighbour loop for one reference particle. */
+static void cm_force_one(const float *particle, int n, int ref, const or_cmodel_params *p,
+                         const cm_consts *c, float *fx_o, float *fy_o, float *fz_o,
+                         float *e_o, int64_t *npairs)
+{
+    float Evdw_acc = 0, Fx_acc = 0, Fy_acc = 0, Fz_acc = 0;
+    float ref_x = particle[11 * ref + 0];
+    float ref_y = particle[11 * ref + 1];
+    float ref_z = particle[11 * ref + 2];
+    int64_t cnt = 0;
+    for (int nb = 0; nb < n; ++nb) {
+        float dx = ref_x - particle[11 * nb + 0];
+        float dy = ref_y - particle[11 * nb + 1];
+        float dz = ref_z - particle[11 * nb + 2];
+        dx = cm_pbc(dx, p->box[0]);
+        dy = cm_pbc(dy, p->box[1]);
+        dz = cm_pbc(dz, p->box[2]);
+        float r2 = dx * dx + dy * dy;                  /* :99 */
+        r2 = r2 + dz * dz;
+        if (r2 > 0 && r2 <= c->rc2) {                  /* :101 */
+            ++cnt;
+            float inv_r2 = 1.0f / r2;                  /* :104 */
+            float inv_r8 = (float)pow((double)inv_r2, 4.0);
+            float inv_r14 = (float)pow((double)inv_r2, 7.0);
+            float inv_r6 = (float)pow((double)inv_r2, 3.0);
+            float inv_r12 = (float)pow((double)inv_r2, 6.0);
+            float vdw12 = c->c4 * c->s12 * inv_r12;    /* :109-112, left to right */
+            float vdw6 = c->c4 * c->s6 * inv_r6;
+            float vdw14 = c->c48 * c->s12 * inv_r14;
+            float vdw8 = c->c24 * c->s6 * inv_r8;
+            float Fvdw = vdw14 - vdw8;                 /* :114-115 */
+            float Evdw = vdw12 - vdw6;
+            float t;
+            t = Fvdw * dx; Fx_acc += t;                /* :117-121 */
+            t = Fvdw * dy; Fy_acc += t;
+            t = Fvdw * dz; Fz_acc += t;
+            Evdw_acc += Evdw;
+        }
+    }
+    *fx_o = Fx_acc; *fy_o = Fy_acc; *fz_o = Fz_acc; *e_o = Evdw_acc;
+    if (npairs) *npairs += cnt;
+}
+
+int or_cmodel_run(float *particle, int n, const or_cmodel_params *p, float *energies,
+                  char *lines, int stop_after_force)
+{
+    cm_consts c;
+    cm_setup(p, &c);
+    const float MASS = p->mass, DT = p->dt;
+    size_t lpos = 0;
+    if (lines) lines[0] = 0;
+    for (int it = 0; it < p->iterations; ++it) {
+        float System_Energy = 0;
+        float System_Kinetic_Energy = 0;
+        for (int ref = 0; ref < n; ++ref) {
+            float Fx, Fy, Fz, E;
+            cm_force_one(particle, n, ref, p, &c, &Fx, &Fy, &Fz, &E, NULL);
+            float *pr = particle + 11 * ref;
+            pr[9] = E;                                 /* :126 */
+            pr[6] = Fx; pr[7] = Fy; pr[8] = Fz;        /* :128-130 */
+            float ax = Fx / MASS, ay = Fy / MASS, az = Fz / MASS;   /* :133-135 */
+            float vx = pr[3], vy = pr[4], vz = pr[5];
+            float t;
+            t = ax * DT; vx += t;                      /* :140-142 */
+            t = ay * DT; vy += t;
+            t = az * DT; vz += t;
+            float v2 = vx * vx + vy * vy;              /* :144: 0.5*MASS in double */
+            v2 = v2 + vz * vz;
+            float Ek = (float)((0.5 * (double)MASS) * (double)v2);
+            pr[10] = Ek;
+            pr[3] = vx; pr[4] = vy; pr[5] = vz;
+            System_Energy += E;                        /* :154-155 */
+            System_Kinetic_Energy += Ek;
+        }
+        System_Energy = (float)((double)System_Energy * 0.5);      /* :158 */
+        if (energies) {
+            energies[2 * it + 0] = System_Energy;
+            energies[2 * it + 1] = System_Kinetic_Energy;
+        }
+        if (lines) {
+            lpos += (size_t)sprintf(lines + lpos,
+                                    "Iteration %d, System energy is %f, Kinetic Energy is %f\n",
+                                    it, (double)System_Energy, (double)System_Kinetic_Energy);
+        }
+        if (stop_after_force) return 1;
+        for (int i = 0; i < n; ++i) {                  /* :164-202 */
+            float *pr = particle + 11 * i;
+            float t;
+            float posx = pr[0], posy = pr[1], posz = pr[2];
+            t = pr[3] * DT; posx += t;
+            t = pr[4] * DT; posy += t;
+            t = pr[5] * DT; posz += t;
+            if (posx < 0) posx += p->box[0]; else if (posx >= p->box[0]) posx -= p->box[0];
+            if (posy < 0) posy += p->box[1]; else if (posy >= p->box[1]) posy -= p->box[1];
+            if (posz < 0) posz += p->box[2]; else if (posz >= p->box[2]) posz -= p->box[2];
+            pr[0] = posx; pr[1] = posy; pr[2] = posz;
+            pr[6] = 0; pr[7] = 0; pr[8] = 0;
+        }
+    }
+    return 1;
+}
+
+int64_t or_cmodel_force_sample(const float *particle, int n, const or_cmodel_params *p,
+                               int nref, float *fout)
+{
+    cm_consts c;
+    cm_setup(p, &c);
+    int64_t cnt = 0;
+    if (nref > n) nref = n;
+    for (int ref = 0; ref < nref; ++ref) {
+        float Fx, Fy, Fz, E;
+        cm_force_one(particle, n, ref, p, &c, &Fx, &Fy, &Fz, &E, &cnt);
+        if (fout) { fout[3 * ref] = Fx; fout[3 * ref + 1] = Fy; fout[3 * ref + 2] = Fz; }
+    }
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* MATLAB binning and the pair stream                                         */
+/* ------------------------------------------------------------------------- */
+
+/* M/Simulation_Verification_LJArgon.m:143-175. */
+int or_bin_ceil(const float *pos, int n, float rc, const int ncell[3], int *cell_xyz_out)
+{
+    int out = 0;
+    for (int i = 0; i < n; ++i) {
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            float v = pos[3 * i + d];
+            c[d] = (v != 0.0f) ? (int)ceilf(v / rc) : 1;   /* single / single, then ceil */
+        }
+        int ok = c[0] > 0 && c[0] <= ncell[0] && c[1] > 0 && c[1] <= ncell[1] &&
+                 c[2] > 0 && c[2] <= ncell[2];
+        for (int d = 0; d < 3; ++d) cell_xyz_out[3 * i + d] = ok ? c[d] : 0;
+        if (!ok) ++out;
+    }
+    return out;
+}
+
+typedef struct { char *buf; int64_t cap, len; } sbuf;
+
+static void sb_put(sbuf *s, const char *fmt, ...)
+    __attribute__((format(printf, 2, 3)));
+#include <stdarg.h>
+static void sb_put(sbuf *s, const char *fmt, ...)
+{
+    char tmp[512];
+    va_list ap;
+    va_start(ap, fmt);
+    int k = vsnprintf(tmp, sizeof tmp, fmt, ap);
+    va_end(ap);
+    if (s->len + k <= s->cap) memcpy(s->buf + s->len, tmp, (size_t)k);
+    s->len += k;
+}
+
+/* M/Simulation_Verification_LJArgon.m:125-175 (load, min-shift, bin) and
+ * :250-572 (filter reservoirs and the pair stream text). */
+int64_t or_pair_stream_text(const double *raw_xyz, int n, float rc, const int ncell[3],
+                            const int home[3], int cell_particle_max, char *buf,
+                            int64_t cap)
+{
+    /* :136-143 min-shift in double, stored as single */
+    double mn[3] = {raw_xyz[0], raw_xyz[1], raw_xyz[2]};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d)
+            if (raw_xyz[3 * i + d] < mn[d]) mn[d] = raw_xyz[3 * i + d];
+    float *pos = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+    int *cxyz = (int *)malloc(sizeof(int) * 3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) pos[3 * i + d] = (float)(raw_xyz[3 * i + d] - mn[d]);
+    or_bin_ceil(pos, n, rc, ncell, cxyz);
+
+    /* cell_particle(cell_id, k, 1:3): stable input order */
+    int ncells = ncell[0] * ncell[1] * ncell[2];
+    int *count = (int *)calloc((size_t)ncells, sizeof(int));
+    float *cellpos = (float *)calloc((size_t)ncells * (size_t)cell_particle_max * 3, sizeof(float));
+    for (int i = 0; i < n; ++i) {
+        if (cxyz[3 * i] == 0) continue;
+        int cid = ((cxyz[3 * i] - 1) * ncell[1] + (cxyz[3 * i + 1] - 1)) * ncell[2] + (cxyz[3 * i + 2] - 1);
+        int k = count[cid]++;
+        if (k < cell_particle_max) memcpy(cellpos + ((size_t)cid * cell_particle_max + k) * 3, pos + 3 * i, 12);
+    }
+
+    /* filter -> cells, :250-510 (offsets relative to the home cell) */
+    static const int fmap[8][2][3] = {
+        {{0, 0, 0}, {9, 9, 9}},   {{0, 0, 1}, {9, 9, 9}},
+        {{0, 1, -1}, {0, 1, 0}},  {{0, 1, 1}, {1, -1, -1}},
+        {{1, -1, 0}, {1, -1, 1}}, {{1, 0, -1}, {1, 0, 0}},
+        {{1, 0, 1}, {1, 1, -1}},  {{1, 1, 0}, {1, 1, 1}},
+    };
+    int RES = 2 * cell_particle_max;
+    float *res = (float *)calloc((size_t)8 * RES * 7, sizeof(float));
+    int fnum[8] = {0};
+    for (int f = 0; f < 8; ++f) {
+        int base = 0;
+        for (int part = 0; part < 2; ++part) {
+            if (fmap[f][part][0] == 9) break;
+            int cx = home[0] + fmap[f][part][0], cy = home[1] + fmap[f][part][1],
+                cz = home[2] + fmap[f][part][2];
+            int cid = ((cx - 1) * ncell[1] + (cy - 1)) * ncell[2] + (cz - 1);
+            int m = count[cid];
+            for (int k = 0; k < m; ++k) {
+                float *r = res + ((size_t)f * RES + base + k) * 7;
+                memcpy(r, cellpos + ((size_t)cid * cell_particle_max + k) * 3, 12);
+                r[3] = (float)cx; r[4] = (float)cy; r[5] = (float)cz; r[6] = (float)(k + 1);
+            }
+            base += m;
+        }
+        fnum[f] = base;
+    }
+    int fmax = 0;
+    for (int f = 0; f < 8; ++f) if (fnum[f] > fmax) fmax = fnum[f];
+
+    sbuf s = {buf, cap, 0};
+    sb_put(&s, "Ref ID\tNeighbor ID\t\t\t\t\t\t\t\t\tValid\tReference Particle Position\t\tNeighbor Particle Position\r\n");
+    int hid = ((home[0] - 1) * ncell[1] + (home[1] - 1)) * ncell[2] + (home[2] - 1);
+    for (int r = 0; r < count[hid]; ++r) {
+        const float *rp = cellpos + ((size_t)hid * cell_particle_max + r) * 3;
+        for (int k = 0; k < fmax; ++k) {
+            sb_put(&s, "%d%d%d%2X\t", home[0], home[1], home[2], r + 1);
+            float tmp[8][3];
+            int valid[8];
+            for (int f = 7; f >= 0; --f) {
+                int src = (k < fnum[f]) ? k : fnum[f] - 1;
+                valid[f] = k < fnum[f];
+                memcpy(tmp[f], res + ((size_t)f * RES + src) * 7, 12);
+                const float *idr = res + ((size_t)f * RES + k) * 7;
+                sb_put(&s, "%d%d%d%2X", (int)idr[3], (int)idr[4], (int)idr[5], (unsigned)idr[6]);
+            }
+            sb_put(&s, "\t");
+            for (int f = 7; f >= 0; --f) sb_put(&s, "%d", valid[f]);
+            sb_put(&s, "\t%08X%08X%08X\t", f2u(rp[2]), f2u(rp[1]), f2u(rp[0]));
+            for (int f = 7; f >= 0; --f)
+                sb_put(&s, "%08X%08X%08X", f2u(tmp[f][2]), f2u(tmp[f][1]), f2u(tmp[f][0]));
+            sb_put(&s, "\r\n");
+        }
+    }
+    free(pos); free(cxyz); free(count); free(cellpos); free(res);
+    return s.len <= cap ? s.len : -s.len;
+}
+
+/* ------------------------------------------------------------------------- */
+/* half-shell table forces, int64 fixed point                                 */
+/* ------------------------------------------------------------------------- */
+
+int or_cell_of(float x, float inv_edge, int nc)
+{
+    float t = x * inv_edge;
+    t = fminf(fmaxf(t, 0.0f), (float)(nc - 1));
+    return (int)t;
+}
+
+int64_t or_fix(float f)
+{
+    double d = (double)f * 4294967296.0;   /* exact: 24-bit mantissa times 2^32 */
+    if (d != d) return 0;
+    if (d > 0x1p62) d = 0x1p62;
+    if (d < -0x1p62) d = -0x1p62;
+    return (int64_t)llrint(d);
+}
+
+/* Half-shell neighbour offsets (dx, dy, dz) of the reference's 13 neighbour cells
+ * {223,231,232,233,311,312,313,321,322,323,331,332,333} around home 222,
+ * D/Particle_Pair_Gen_HalfShell_64_Cells.v:213-226, preceded by the home cell. */
+static const int HS[14][3] = {
+    {0, 0, 0},  {0, 0, 1},  {0, 1, -1}, {0, 1, 0},  {0, 1, 1},
+    {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},
+    {1, 0, 1},  {1, 1, -1}, {1, 1, 0},  {1, 1, 1},
+};
+
+typedef struct {
+    int ncells;
+    int *start, *count, *list;
+} cell_list;
+
+static void build_cells(int n, const float *pos4, const float box[3], const int nc[3],
+                        cell_list *cl)
+{
+    float inv[3];
+    for (int d = 0; d < 3; ++d) inv[d] = (float)((double)nc[d] / (double)box[d]);
+    cl->ncells = nc[0] * nc[1] * nc[2];
+    cl->start = (int *)calloc((size_t)cl->ncells + 1, sizeof(int));
+    cl->count = (int *)calloc((size_t)cl->ncells, sizeof(int));
+    cl->list = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *cid = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) {
+        int cx = or_cell_of(pos4[4 * i + 0], inv[0], nc[0]);
+        int cy = or_cell_of(pos4[4 * i + 1], inv[1], nc[1]);
+        int cz = or_cell_of(pos4[4 * i + 2], inv[2], nc[2]);
+        cid[i] = (cx * nc[1] + cy) * nc[2] + cz;
+        cl->count[cid[i]]++;
+    }
+    for (int c = 0; c < cl->ncells; ++c) cl->start[c + 1] = cl->start[c] + cl->count[c];
+    int *fill = (int *)calloc((size_t)cl->ncells, sizeof(int));
+    for (int i = 0; i < n; ++i) cl->list[cl->start[cid[i]] + fill[cid[i]]++] = i;
+    free(fill);
+    free(cid);
+}
+
+static void free_cells(cell_list *cl) { free(cl->start); free(cl->count); free(cl->list); }
+
+/* Scalar force of one accepted pair (the kernel's per-pair arithmetic). */
+static inline float pair_scalar(const or_rl_system *s, float r2, int ti, int tj, float qq)
+{
+    float A = s->pair_a[ti * s->ntypes + tj];
+    float B = s->pair_b[ti * s->ntypes + tj];
+    const int par = s->mode & OR_MODE_PARITY;
+    const int coul = s->mode & OR_MODE_COULOMB;
+    if (s->mode & OR_MODE_ANALYTIC) {
+        float ir2 = 1.0f / r2;
+        float ir6 = ir2 * ir2;
+        ir6 = ir6 * ir2;
+        float ir8 = ir6 * ir2;
+        float ir14 = ir8 * ir6;
+        float F = fmaf(A, ir14, -(B * ir8));
+        if (coul) {
+            float ir3 = ir2 * sqrtf(ir2);
+            ir3 = ir3 * 0.0796178343949044585987261f;   /* 1/(4*3.14) as float */
+            F = fmaf(qq, ir3, F);
+        }
+        return F;
+    }
+    int idx = or_lut_index(r2, s->exp_offset, s->nseg);
+    const float *c = s->tab + 6 * (size_t)idx;
+    float F;
+    if (par) {
+        float t14 = c[1] * r2; t14 = t14 + c[0];
+        float t8 = c[3] * r2;  t8 = t8 + c[2];
+        float a = A * t14, b = B * t8;
+        F = a - b;
+        if (coul) {
+            float t3 = c[5] * r2; t3 = t3 + c[4];
+            float q3 = qq * t3;
+            F = F + q3;
+        }
+    } else {
+        float t14 = fmaf(c[1], r2, c[0]);
+        float t8 = fmaf(c[3], r2, c[2]);
+        F = fmaf(A, t14, -(B * t8));
+        if (coul) {
+            float t3 = fmaf(c[5], r2, c[4]);
+            F = fmaf(qq, t3, F);
+        }
+    }
+    return F;
+}
+
+int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
+{
+    const int n = s->n;
+    cell_list cl;
+    build_cells(n, s->pos4, s->box, s->ncell, &cl);
+    memset(fix_out, 0, sizeof(int64_t) * 3 * (size_t)n);
+    uint64_t *acc = (uint64_t *)fix_out;   /* two's-complement wrap-around adds */
+    const int par = s->mode & OR_MODE_PARITY;
+    int64_t npairs = 0;
+    const int *nc = s->ncell;
+    for (int hx = 0; hx < nc[0]; ++hx)
+    for (int hy = 0; hy < nc[1]; ++hy)
+    for (int hz = 0; hz < nc[2]; ++hz) {
+        int h = (hx * nc[1] + hy) * nc[2] + hz;
+        for (int o = 0; o < 14; ++o) {
+            int raw[3] = {hx + HS[o][0], hy + HS[o][1], hz + HS[o][2]};
+            float S[3];
+            int nb[3];
+            for (int d = 0; d < 3; ++d) {
+                if (raw[d] < 0) { nb[d] = raw[d] + nc[d]; S[d] = -s->box[d]; }
+                else if (raw[d] >= nc[d]) { nb[d] = raw[d] - nc[d]; S[d] = s->box[d]; }
+                else { nb[d] = raw[d]; S[d] = 0.0f; }
+            }
+            int c = (nb[0] * nc[1] + nb[1]) * nc[2] + nb[2];
+            for (int a = 0; a < cl.count[h]; ++a) {
+                int i = cl.list[cl.start[h] + a];
+                const float *pi = s->pos4 + 4 * (size_t)i;
+                int b0 = (o == 0) ? a + 1 : 0;
+                for (int b = b0; b < cl.count[c]; ++b) {
+                    int j = cl.list[cl.start[c] + b];
+                    const float *pj = s->pos4 + 4 * (size_t)j;
+                    float dx, dy, dz, r2;
+                    if (par) {
+                        dx = pi[0] - pj[0]; dx = dx - S[0];
+                        dy = pi[1] - pj[1]; dy = dy - S[1];
+                        dz = pi[2] - pj[2]; dz = dz - S[2];
+                        r2 = dx * dx + dy * dy;
+                        r2 = r2 + dz * dz;
+                    } else {
+                        float xj = pj[0] + S[0], yj = pj[1] + S[1], zj = pj[2] + S[2];
+                        dx = pi[0] - xj; dy = pi[1] - yj; dz = pi[2] - zj;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    }
+                    if (!(r2 > 0.0f && r2 <= s->rc2)) continue;
+                    if (s->excl_mask && s->excl_mask[i] && s->excl_mask[j]) { /* reserved */ }
+                    ++npairs;
+                    float qq = pi[3] * pj[3];
+                    float F = pair_scalar(s, r2, s->type[i], s->type[j], qq);
+                    int64_t kx = or_fix(F * dx), ky = or_fix(F * dy), kz = or_fix(F * dz);
+                    acc[3 * i + 0] += (uint64_t)kx; acc[3 * j + 0] -= (uint64_t)kx;
+                    acc[3 * i + 1] += (uint64_t)ky; acc[3 * j + 1] -= (uint64_t)ky;
+                    acc[3 * i + 2] += (uint64_t)kz; acc[3 * j + 2] -= (uint64_t)kz;
+                }
+            }
+        }
+    }
+    free_cells(&cl);
+    return npairs;
+}
+
+/* FP64 evaluation of the same table model: FP32 coefficients, everything else in
+ * double, the same pair set (exact images). */
+int64_t or_table_fp64(const or_rl_system *s, double *f_out)
+{
+    const int n = s->n;
+    cell_list cl;
+    build_cells(n, s->pos4, s->box, s->ncell, &cl);
+    memset(f_out, 0, sizeof(double) * 3 * (size_t)n);
+    int64_t npairs = 0;
+    const int *nc = s->ncell;
+    for (int h = 0; h < cl.ncells; ++h) {
+        int hx = h / (nc[1] * nc[2]), hy = (h / nc[2]) % nc[1], hz = h % nc[2];
+        for (int o = 0; o < 14; ++o) {
+            int raw[3] = {hx + HS[o][0], hy + HS[o][1], hz + HS[o][2]};
+            double S[3];
+            int nb[3];
+            for (int d = 0; d < 3; ++d) {
+                if (raw[d] < 0) { nb[d] = raw[d] + nc[d]; S[d] = -(double)s->box[d]; }
+                else if (raw[d] >= nc[d]) { nb[d] = raw[d] - nc[d]; S[d] = s->box[d]; }
+                else { nb[d] = raw[d]; S[d] = 0.0; }
+            }
+            int c = (nb[0] * nc[1] + nb[1]) * nc[2] + nb[2];
+            for (int a = 0; a < cl.count[h]; ++a) {
+                int i = cl.list[cl.start[h] + a];
+                const float *pi = s->pos4 + 4 * (size_t)i;
+                int b0 = (o == 0) ? a + 1 : 0;
+                for (int b = b0; b < cl.count[c]; ++b) {
+                    int j = cl.list[cl.start[c] + b];
+                    const float *pj = s->pos4 + 4 * (size_t)j;
+                    double dx = (double)pi[0] - (double)pj[0] - S[0];
+                    double dy = (double)pi[1] - (double)pj[1] - S[1];
+                    double dz = (double)pi[2] - (double)pj[2] - S[2];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    if (!(r2 > 0.0 && r2 <= (double)s->rc2)) continue;
+                    ++npairs;
+                    int idx = or_lut_index((float)r2, s->exp_offset, s->nseg);
+                    const float *cf = s->tab + 6 * (size_t)idx;
+                    int ti = s->type[i], tj = s->type[j];
+                    double A = s->pair_a[ti * s->ntypes + tj], B = s->pair_b[ti * s->ntypes + tj];
+                    double F = A * ((double)cf[1] * r2 + cf[0]) - B * ((double)cf[3] * r2 + cf[2]);
+                    if (s->mode & OR_MODE_COULOMB)
+                        F += (double)pi[3] * (double)pj[3] * ((double)cf[5] * r2 + cf[4]);
+                    F *= 0.1;   /* kJ/mol/nm multipliers -> kJ/mol/A */
+                    f_out[3 * i] += F * dx; f_out[3 * j] -= F * dx;
+                    f_out[3 * i + 1] += F * dy; f_out[3 * j + 1] -= F * dy;
+                    f_out[3 * i + 2] += F * dz; f_out[3 * j + 2] -= F * dz;
+                }
+            }
+        }
+    }
+    free_cells(&cl);
+    return npairs;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 analytic direct oracle                                                */
+/* ------------------------------------------------------------------------- */
+
+static inline void lj_coul_fp64(double r2, double eps, double sig, double qq, double ke,
+                                int coulomb, double *F, double *E)
+{
+    double ir2 = 1.0 / r2;
+    double s2 = sig * sig * ir2;
+    double s6 = s2 * s2 * s2;
+    double s12 = s6 * s6;
+    *F = (48.0 * eps * s12 - 24.0 * eps * s6) * ir2;   /* -dU/dr / r */
+    *E = 4.0 * eps * (s12 - s6);
+    if (coulomb) {
+        double ir = sqrt(ir2);
+        *F += ke * qq * ir * ir2;
+        *E += ke * qq * ir;
+    }
+}
+
+int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *q,
+                       int ntypes, const double *eps_ij, const double *sig_ij,
+                       double ke, int coulomb, const double box[3], double rc,
+                       int use_cells, double *f_out, double *energy, double *virial)
+{
+    const double rc2 = rc * rc;
+    int64_t npairs = 0;
+    double etot = 0.0, vir[9] = {0};
+    memset(f_out, 0, sizeof(double) * 3 * (size_t)n);
+    if (!use_cells) {
+        /* every image within rc: image range per axis */
+        int im[3];
+        for (int d = 0; d < 3; ++d) im[d] = (int)ceil(rc / box[d]) + 1;
+        for (int i = 0; i < n; ++i) {
+            double fi[3] = {0, 0, 0};
+            for (int j = 0; j < n; ++j) {
+                for (int ax = -im[0]; ax <= im[0]; ++ax)
+                for (int ay = -im[1]; ay <= im[1]; ++ay)
+                for (int az = -im[2]; az <= im[2]; ++az) {
+                    if (i == j && ax == 0 && ay == 0 && az == 0) continue;
+                    double dx = xyz[3 * i] - xyz[3 * j] - ax * box[0];
+                    double dy = xyz[3 * i + 1] - xyz[3 * j + 1] - ay * box[1];
+                    double dz = xyz[3 * i + 2] - xyz[3 * j + 2] - az * box[2];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 > rc2 || r2 <= 0.0) continue;
+                    int tt = type[i] * ntypes + type[j];
+                    double F, E;
+                    lj_coul_fp64(r2, eps_ij[tt], sig_ij[tt], q ? q[i] * q[j] : 0.0, ke,
+                                 coulomb, &F, &E);
+                    fi[0] += F * dx; fi[1] += F * dy; fi[2] += F * dz;
+                    etot += 0.5 * E;
+                    double d3[3] = {dx, dy, dz};
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) vir[3 * a + b] += 0.5 * F * d3[a] * d3[b];
+                    ++npairs;
+                }
+            }
+            f_out[3 * i] = fi[0]; f_out[3 * i + 1] = fi[1]; f_out[3 * i + 2] = fi[2];
+        }
+        npairs /= 2;
+    } else {
+        int nc[3];
+        for (int d = 0; d < 3; ++d) { nc[d] = (int)floor(box[d] / rc); if (nc[d] < 1) nc[d] = 1; }
+        int ncells = nc[0] * nc[1] * nc[2];
+        int *start = (int *)calloc((size_t)ncells + 1, sizeof(int));
+        int *cid = (int *)malloc(sizeof(int) * (size_t)n);
+        int *list = (int *)malloc(sizeof(int) * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            int c3[3];
+            for (int d = 0; d < 3; ++d) {
+                double x = xyz[3 * i + d] - box[d] * floor(xyz[3 * i + d] / box[d]);
+                int c = (int)(x / box[d] * nc[d]);
+                if (c >= nc[d]) c = nc[d] - 1;
+                if (c < 0) c = 0;
+                c3[d] = c;
+            }
+            cid[i] = (c3[0] * nc[1] + c3[1]) * nc[2] + c3[2];
+            start[cid[i] + 1]++;
+        }
+        for (int c = 0; c < ncells; ++c) start[c + 1] += start[c];
+        int *fill = (int *)calloc((size_t)ncells, sizeof(int));
+        for (int i = 0; i < n; ++i) list[start[cid[i]] + fill[cid[i]]++] = i;
+        free(fill);
+        double et = 0.0;
+        int64_t np = 0;
+        /* full shell: every atom gathers from the 27 neighbour cell images */
+        #pragma omp parallel for schedule(dynamic, 64) reduction(+:et, np)
+        for (int i = 0; i < n; ++i) {
+            int c = cid[i];
+            int cx = c / (nc[1] * nc[2]), cy = (c / nc[2]) % nc[1], cz = c % nc[2];
+            double xi[3];
+            for (int d = 0; d < 3; ++d) xi[d] = xyz[3 * i + d] - box[d] * floor(xyz[3 * i + d] / box[d]);
+            double fi[3] = {0, 0, 0};
+            for (int ox = -1; ox <= 1; ++ox)
+            for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                int r[3] = {cx + ox, cy + oy, cz + oz}, nb[3];
+                double S[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (r[d] < 0) { nb[d] = r[d] + nc[d]; S[d] = -box[d]; }
+                    else if (r[d] >= nc[d]) { nb[d] = r[d] - nc[d]; S[d] = box[d]; }
+                    else { nb[d] = r[d]; S[d] = 0.0; }
+                }
+                int cc = (nb[0] * nc[1] + nb[1]) * nc[2] + nb[2];
+                for (int k = start[cc]; k < start[cc + 1]; ++k) {
+                    int j = list[k];
+                    double xj[3];
+                    for (int d = 0; d < 3; ++d) xj[d] = xyz[3 * j + d] - box[d] * floor(xyz[3 * j + d] / box[d]) + S[d];
+                    double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 > rc2 || r2 <= 0.0) continue;
+                    int tt = type[i] * ntypes + type[j];
+                    double F, E;
+                    lj_coul_fp64(r2, eps_ij[tt], sig_ij[tt], q ? q[i] * q[j] : 0.0, ke,
+                                 coulomb, &F, &E);
+                    fi[0] += F * dx; fi[1] += F * dy; fi[2] += F * dz;
+                    et += 0.5 * E;
+                    ++np;
+                }
+            }
+            f_out[3 * i] = fi[0]; f_out[3 * i + 1] = fi[1]; f_out[3 * i + 2] = fi[2];
+        }
+        etot = et;
+        npairs = np / 2;
+        free(start); free(cid); free(list);
+    }
+    if (energy) *energy = etot;
+    if (virial) memcpy(virial, vir, sizeof vir);
+    return npairs;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Motion_Update                                                              */
+/* ------------------------------------------------------------------------- */
+
+static inline float wrap1(float x, float L)
+{
+    if (x < 0.0f) x = x + L;
+    if (x >= L) x = x - L;
+    return x;
+}
+
+void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,
+                      const int *type, const double *hk, const double *mass, float dt,
+                      const float box[3], int first, int kick_drift, double *ke_out)
+{
+    double ke = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double k = hk[type[i]];
+        float v2 = 0.0f;
+        double vv = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            float dv = (float)((double)fix[3 * i + d] * k);
+            float v = vel3[3 * i + d];
+            float vn;
+            if (kick_drift) {
+                float dv2 = dv + dv;
+                vn = v + dv2;
+                vv += (double)vn * (double)vn;
+            } else {
+                float vf = first ? v : v + dv;
+                vv += (double)vf * (double)vf;
+                vn = vf + dv;
+            }
+            float step = vn * dt;
+            float x = pos4[4 * i + d] + step;
+            pos4[4 * i + d] = wrap1(x, box[d]);
+            vel3[3 * i + d] = vn;
+        }
+        (void)v2;
+        ke += 0.5 * mass[type[i]] * vv * 1.0e4;
+    }
+    if (ke_out) *ke_out = ke;
+}

@@ -1,0 +1,290 @@
+"""ctypes wrapper of liboracle.so, the CPU restatement of the reference's range-limited path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by bench.py's CPU
+baseline leg, never by the product package.  See oracle.h for the reference file:line each
+function restates and for what pins it.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "md_eval_model_ref")
+
+MODE_PARITY, MODE_COULOMB, MODE_ANALYTIC = 1, 2, 4
+
+_lib = None
+
+F = ctypes.POINTER(ctypes.c_float)
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+I64 = ctypes.POINTER(ctypes.c_int64)
+
+
+class CModelParams(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_float), ("sigma", ctypes.c_float), ("mass", ctypes.c_float),
+                ("dt", ctypes.c_float), ("rc", ctypes.c_float), ("box", ctypes.c_float * 3),
+                ("iterations", ctypes.c_int)]
+
+
+class RLSystem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("box", ctypes.c_float * 3), ("ncell", ctypes.c_int * 3),
+                ("rc2", ctypes.c_float), ("pos4", F), ("type", I), ("ntypes", ctypes.c_int),
+                ("pair_a", F), ("pair_b", F), ("tab", F), ("exp_offset", ctypes.c_int),
+                ("nseg", ctypes.c_int), ("mode", ctypes.c_int), ("excl_mask", ctypes.c_void_p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_lut_index.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        L.or_pair_golden.argtypes = [F, F, F, F, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_int, F, F, ctypes.c_int64, F]
+        L.or_pair_golden.restype = None
+        L.or_cmodel_default_params.argtypes = [ctypes.POINTER(CModelParams)]
+        L.or_cmodel_default_params.restype = None
+        L.or_cmodel_run.argtypes = [F, ctypes.c_int, ctypes.POINTER(CModelParams), F,
+                                    ctypes.c_char_p, ctypes.c_int]
+        L.or_cmodel_force_sample.argtypes = [F, ctypes.c_int, ctypes.POINTER(CModelParams),
+                                             ctypes.c_int, F]
+        L.or_cmodel_force_sample.restype = ctypes.c_int64
+        L.or_bin_ceil.argtypes = [F, ctypes.c_int, ctypes.c_float, I, I]
+        L.or_pair_stream_text.argtypes = [D, ctypes.c_int, ctypes.c_float, I, I, ctypes.c_int,
+                                          ctypes.c_char_p, ctypes.c_int64]
+        L.or_pair_stream_text.restype = ctypes.c_int64
+        L.or_cell_of.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.or_fix.argtypes = [ctypes.c_float]
+        L.or_fix.restype = ctypes.c_int64
+        L.or_rl_forces_fix.argtypes = [ctypes.POINTER(RLSystem), I64]
+        L.or_rl_forces_fix.restype = ctypes.c_int64
+        L.or_table_fp64.argtypes = [ctypes.POINTER(RLSystem), D]
+        L.or_table_fp64.restype = ctypes.c_int64
+        L.or_fp64_direct.argtypes = [ctypes.c_int, D, I, D, ctypes.c_int, D, D, ctypes.c_double,
+                                     ctypes.c_int, D, ctypes.c_double, ctypes.c_int, D, D, D]
+        L.or_fp64_direct.restype = ctypes.c_int64
+        L.or_motion_update.argtypes = [ctypes.c_int, F, F, I64, I, D, D, ctypes.c_float, F,
+                                       ctypes.c_int, ctypes.c_int, D]
+        L.or_motion_update.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------------------- tables
+def lut_index(r2, exp_offset=121, nseg=14):
+    return lib().or_lut_index(float(np.float32(r2)), exp_offset, nseg)
+
+
+def pair_golden(c0_14, c1_14, c0_8, c1_8, ref, nbr, rc2, exp_offset=121, nseg=14, strict=True):
+    ref = np.ascontiguousarray(ref, np.float32).reshape(-1, 3)
+    nbr = np.ascontiguousarray(nbr, np.float32).reshape(-1, 3)
+    tabs = [np.ascontiguousarray(t, np.float32) for t in (c0_14, c1_14, c0_8, c1_8)]
+    out = np.zeros((ref.shape[0], 8), np.float32)
+    lib().or_pair_golden(*[_p(t, F) for t in tabs], exp_offset, nseg, float(rc2), int(strict),
+                         _p(ref, F), _p(nbr, F), ref.shape[0], _p(out, F))
+    return out
+
+
+# ---------------------------------------------------------------------------- C model
+def cmodel_default_params():
+    p = CModelParams()
+    lib().or_cmodel_default_params(ctypes.byref(p))
+    return p
+
+
+def cmodel_run(particle, params, with_lines=False):
+    a = np.ascontiguousarray(particle, np.float32).copy()
+    n = a.shape[0]
+    e = np.zeros(2 * params.iterations, np.float32)
+    buf = ctypes.create_string_buffer(128 * params.iterations + 16) if with_lines else None
+    lib().or_cmodel_run(_p(a, F), n, ctypes.byref(params), _p(e, F), buf, 0)
+    lines = buf.value.decode() if with_lines else None
+    return a, e.reshape(-1, 2), lines
+
+
+def cmodel_force_sample(particle, params, nref):
+    a = np.ascontiguousarray(particle, np.float32)
+    f = np.zeros((nref, 3), np.float32)
+    cnt = lib().or_cmodel_force_sample(_p(a, F), a.shape[0], ctypes.byref(params), nref, _p(f, F))
+    return cnt, f
+
+
+# ---------------------------------------------------------------------------- binning
+def bin_ceil(pos, rc, ncell):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    nc = np.ascontiguousarray(ncell, np.int32)
+    out = np.zeros((pos.shape[0], 3), np.int32)
+    nout = lib().or_bin_ceil(_p(pos, F), pos.shape[0], float(rc), _p(nc, I), _p(out, I))
+    return out, nout
+
+
+def pair_stream_text(raw_xyz, rc, ncell, home, cell_particle_max=100):
+    raw = np.ascontiguousarray(raw_xyz, np.float64).reshape(-1, 3)
+    nc = np.ascontiguousarray(ncell, np.int32)
+    hm = np.ascontiguousarray(home, np.int32)
+    cap = 1 << 20
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        n = lib().or_pair_stream_text(_p(raw, D), raw.shape[0], float(rc), _p(nc, I), _p(hm, I),
+                                      cell_particle_max, buf, cap)
+        if n >= 0:
+            return buf.raw[:n]
+        cap = -n + 16
+
+
+def cell_of(x, inv_edge, nc):
+    return lib().or_cell_of(float(x), float(inv_edge), nc)
+
+
+def fix(f):
+    return lib().or_fix(float(np.float32(f)))
+
+
+# ---------------------------------------------------------------------------- host-side setup
+# The same host arithmetic as libmdrl's md_rl_create (Lorentz-Berthelot in double, rounded
+# once to float; forces in kJ/mol/nm, hence the factor 10).
+def pair_multipliers(eps, sigma, analytic=False):
+    nt = len(eps)
+    a = np.zeros(nt * nt, np.float32)
+    b = np.zeros(nt * nt, np.float32)
+    for i in range(nt):
+        for j in range(nt):
+            sg = 0.5 * (float(np.float32(sigma[i])) + float(np.float32(sigma[j])))
+            ep = math.sqrt(float(np.float32(eps[i])) * float(np.float32(eps[j])))
+            s6 = math.pow(sg, 6.0)
+            s12 = s6 * s6
+            if analytic:
+                a[i * nt + j] = np.float32(480.0 * ep * s12)
+                b[i * nt + j] = np.float32(240.0 * ep * s6)
+            else:
+                a[i * nt + j] = np.float32(10.0 * ep * s12)
+                b[i * nt + j] = np.float32(10.0 * ep * s6)
+    return a, b
+
+
+def charge_scale(coulomb_k):
+    return math.sqrt(10.0 * float(np.float32(coulomb_k)) * 4.0 * 3.14) if coulomb_k else 0.0
+
+
+def half_kick(mass, dt):
+    return np.array([math.ldexp(1.0e-5 / float(np.float32(m)) * (0.5 * float(np.float32(dt))), -32)
+                     for m in mass], np.float64)
+
+
+def interleave(tabs):
+    """tabs: list of 6 arrays c0_a, c1_a, c0_b, c1_b, c0_c, c1_c (None -> zeros)."""
+    n = len(tabs[0])
+    out = np.zeros((n, 6), np.float32)
+    for k, t in enumerate(tabs):
+        if t is not None:
+            out[:, k] = t
+    return out
+
+
+class RL:
+    """An or_rl_system with owned arrays."""
+
+    def __init__(self, pos_xyz, box, ncell, rc, types, eps, sigma, tab6, charge=None,
+                 coulomb_k=0.0, mode=0, exp_offset=121, nseg=14):
+        n = pos_xyz.shape[0]
+        self.pos4 = np.zeros((n, 4), np.float32)
+        self.pos4[:, :3] = pos_xyz
+        qs = charge_scale(coulomb_k)
+        if charge is not None and coulomb_k:
+            self.pos4[:, 3] = (np.asarray(charge, np.float32).astype(np.float64) * qs).astype(np.float32)
+            mode |= MODE_COULOMB
+        self.type = np.ascontiguousarray(types, np.int32)
+        self.pa, self.pb = pair_multipliers(eps, sigma, analytic=bool(mode & MODE_ANALYTIC))
+        self.tab = np.ascontiguousarray(tab6, np.float32)
+        s = RLSystem()
+        s.n = n
+        s.box[:] = [float(np.float32(b)) for b in box]
+        s.ncell[:] = list(ncell)
+        s.rc2 = float(np.float32(rc) * np.float32(rc))
+        s.pos4, s.type = _p(self.pos4, F), _p(self.type, I)
+        s.ntypes = len(eps)
+        s.pair_a, s.pair_b, s.tab = _p(self.pa, F), _p(self.pb, F), _p(self.tab, F)
+        s.exp_offset, s.nseg, s.mode = exp_offset, nseg, mode
+        s.excl_mask = None
+        self.s = s
+        self.n = n
+
+    def forces_fix(self):
+        out = np.zeros((self.n, 3), np.int64)
+        npairs = lib().or_rl_forces_fix(ctypes.byref(self.s), _p(out, I64))
+        return out, npairs
+
+    def forces_table_fp64(self):
+        out = np.zeros((self.n, 3), np.float64)
+        npairs = lib().or_table_fp64(ctypes.byref(self.s), _p(out, D))
+        return out, npairs
+
+
+def fp64_direct(xyz, types, eps, sigma, box, rc, charge=None, ke=0.0, use_cells=True):
+    """Analytic FP64 forces (kJ/mol/A), LJ with Lorentz-Berthelot + cutoff Coulomb."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    n = xyz.shape[0]
+    nt = len(eps)
+    e = np.zeros(nt * nt)
+    s = np.zeros(nt * nt)
+    for i in range(nt):
+        for j in range(nt):
+            s[i * nt + j] = 0.5 * (float(sigma[i]) + float(sigma[j]))
+            e[i * nt + j] = math.sqrt(float(eps[i]) * float(eps[j]))
+    t = np.ascontiguousarray(types, np.int32)
+    q = None if charge is None else np.ascontiguousarray(charge, np.float64)
+    bx = np.ascontiguousarray(box, np.float64)
+    f = np.zeros((n, 3))
+    en = ctypes.c_double()
+    vir = np.zeros(9)
+    npairs = lib().or_fp64_direct(n, _p(xyz, D), _p(t, I), None if q is None else _p(q, D), nt,
+                                  _p(e, D), _p(s, D), float(ke), 1 if (q is not None and ke) else 0,
+                                  _p(bx, D), float(rc), 1 if use_cells else 0, _p(f, D),
+                                  ctypes.byref(en), _p(vir, D))
+    return f, en.value, vir.reshape(3, 3), npairs
+
+
+def motion_update(pos4, vel3, fixf, types, hk, mass, dt, box, first, kick_drift=False):
+    pos4 = np.ascontiguousarray(pos4, np.float32).copy()
+    vel3 = np.ascontiguousarray(vel3, np.float32).copy()
+    fixf = np.ascontiguousarray(fixf, np.int64)
+    t = np.ascontiguousarray(types, np.int32)
+    hk = np.ascontiguousarray(hk, np.float64)
+    ms = np.ascontiguousarray(mass, np.float64)
+    bx = np.ascontiguousarray(box, np.float32)
+    ke = ctypes.c_double()
+    lib().or_motion_update(pos4.shape[0], _p(pos4, F), _p(vel3, F), _p(fixf, I64), _p(t, I),
+                           _p(hk, D), _p(ms, D), float(dt), _p(bx, F), int(first),
+                           int(kick_drift), ctypes.byref(ke))
+    return pos4, vel3, ke.value
+
+
+def run_reference_cmodel(pdb_text, workdir):
+    """Runs the reference C model (built under _ref/ from the reference sources) on a PDB.
+
+    The model opens the relative path below (R/C_Model_ForceEval/MD_Evaluation_Model.cpp:30-32);
+    the PDB is written there inside workdir.  Returns the printed lines."""
+    rel = ("C:/Users/Ethan/Desktop/WorkingFolder/Ethan_RL_Pipeline_1st_Order_SingleFloat_18.0/"
+           "SourceCode/MatlabScripts")
+    d = os.path.join(workdir, rel)
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ar_gas.pdb"), "w") as f:
+        f.write(pdb_text)
+    r = subprocess.run([REF_BIN], cwd=workdir, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("reference C model failed: %s" % r.stderr)
+    return r.stdout

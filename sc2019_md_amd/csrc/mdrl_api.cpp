@@ -1,0 +1,734 @@
+// mdrl_api.cpp -- the C ABI: pair batches, systems (contexts) and the C-model entry.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "mdrl_internal.h"
+
+using namespace mdrl;
+
+// ---------------------------------------------------------------------------------
+// device checks and device tables
+// ---------------------------------------------------------------------------------
+
+static int check_device()
+{
+    static int ok = 0;   // 1 = gfx950 present
+    if (ok) return MDRL_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device visible");
+        return MDRL_ENODEV;
+    }
+    int dev = 0;
+    MDRL_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MDRL_HIP(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("device %d is %s; libmdrl is built for gfx950 only", dev, prop.gcnArchName);
+        return MDRL_ENODEV;
+    }
+    ok = 1;
+    return MDRL_OK;
+}
+
+// Interleaved device copy of three terms: 6 floats per entry (c0_a, c1_a, c0_b, c1_b,
+// c0_c, c1_c); absent terms are zeros.
+static std::vector<float> interleave(const md_rl_table *t, int ta, int tb, int tc)
+{
+    std::vector<float> v((size_t)t->nent * 6, 0.0f);
+    const int terms[3] = {ta, tb, tc};
+    for (int e = 0; e < t->nent; ++e)
+        for (int k = 0; k < 3; ++k)
+            if (t->has[terms[k]]) {
+                v[6 * (size_t)e + 2 * k] = t->c0[terms[k]][e];
+                v[6 * (size_t)e + 2 * k + 1] = t->c1[terms[k]][e];
+            }
+    return v;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+        return MDRL_ENOMEM;
+    }
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// pair batch
+// ---------------------------------------------------------------------------------
+
+static int eval_pairs_common(const md_rl_table *t, const float *ref, const float *nbr,
+                             const float *pa, const float *pb, const float *pqq, float *fs,
+                             float *fxyz, int64_t n, float rc2, int flags)
+{
+    if (!t || !ref || !nbr || !fxyz || n < 0) { set_error("md_rl_eval_pairs: bad argument"); return MDRL_EINVAL; }
+    if ((flags & MDRL_PAIR_COULOMB) && !t->has[MDRL_TERM_3]) {
+        set_error("md_rl_eval_pairs: Coulomb requested but the table has no r^-3 term");
+        return MDRL_EINVAL;
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    if (n == 0) return MDRL_OK;
+    std::vector<float> tab = interleave(t, MDRL_TERM_14, MDRL_TERM_8, MDRL_TERM_3);
+    float *d_tab = nullptr, *d_ref = nullptr, *d_nbr = nullptr, *d_out = nullptr, *d_fs = nullptr;
+    float *d_pa = nullptr, *d_pb = nullptr, *d_pq = nullptr;
+    int st = MDRL_OK;
+    auto up = [&](float **d, const float *h, size_t cnt) -> int {
+        if (!h) return MDRL_OK;
+        int r = dalloc(d, cnt);
+        if (r) return r;
+        hipError_t e = hipMemcpy(*d, h, sizeof(float) * cnt, hipMemcpyHostToDevice);
+        return e == hipSuccess ? MDRL_OK : hip_fail(e, "hipMemcpy");
+    };
+    if (!st) st = up(&d_tab, tab.data(), tab.size());
+    if (!st) st = up(&d_ref, ref, 3 * (size_t)n);
+    if (!st) st = up(&d_nbr, nbr, 3 * (size_t)n);
+    if (!st) st = up(&d_pa, pa, (size_t)n);
+    if (!st) st = up(&d_pb, pb, (size_t)n);
+    if (!st) st = up(&d_pq, pqq, (size_t)n);
+    if (!st) st = dalloc(&d_out, 3 * (size_t)n);
+    if (!st && fs) st = dalloc(&d_fs, (size_t)n);
+    if (!st) st = launch_eval_pairs(d_tab, t->exp_offset, t->nseg, d_ref, d_nbr, d_pa, d_pb, d_pq,
+                                    d_fs, d_out, n, rc2, flags, 0);
+    if (!st) {
+        hipError_t e = hipMemcpy(fxyz, d_out, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy");
+    }
+    if (!st && fs) {
+        hipError_t e = hipMemcpy(fs, d_fs, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy");
+    }
+    for (float *p : {d_tab, d_ref, d_nbr, d_out, d_fs, d_pa, d_pb, d_pq})
+        if (p) (void)hipFree(p);
+    return st;
+}
+
+extern "C" int md_rl_eval_pairs_ex(const md_rl_table *t, const float *ref_xyz, const float *nbr_xyz,
+                                   const float *p_a, const float *p_b, const float *p_qq,
+                                   float *force_s, float *force_xyz, int64_t npairs, float rc2,
+                                   int flags)
+{
+    return eval_pairs_common(t, ref_xyz, nbr_xyz, p_a, p_b, p_qq, force_s, force_xyz, npairs, rc2, flags);
+}
+
+extern "C" int md_rl_eval_pairs(const md_rl_table *t, const int32_t *ref_id, const int32_t *nbr_id,
+                                const float *ref_x, const float *ref_y, const float *ref_z,
+                                const float *nbr_x, const float *nbr_y, const float *nbr_z,
+                                float *force_x, float *force_y, float *force_z, int64_t npairs,
+                                float rc2, int flags)
+{
+    (void)ref_id; (void)nbr_id;   // carried by the reference's kernel, unused by the force
+    if (npairs < 0 || !ref_x || !ref_y || !ref_z || !nbr_x || !nbr_y || !nbr_z || !force_x ||
+        !force_y || !force_z) {
+        set_error("md_rl_eval_pairs: bad argument");
+        return MDRL_EINVAL;
+    }
+    std::vector<float> r(3 * (size_t)npairs), b(3 * (size_t)npairs), o(3 * (size_t)npairs);
+    for (int64_t p = 0; p < npairs; ++p) {
+        r[3 * p] = ref_x[p]; r[3 * p + 1] = ref_y[p]; r[3 * p + 2] = ref_z[p];
+        b[3 * p] = nbr_x[p]; b[3 * p + 1] = nbr_y[p]; b[3 * p + 2] = nbr_z[p];
+    }
+    int st = eval_pairs_common(t, r.data(), b.data(), nullptr, nullptr, nullptr, nullptr, o.data(),
+                               npairs, rc2, flags & ~MDRL_PAIR_COULOMB);
+    if (st) return st;
+    for (int64_t p = 0; p < npairs; ++p) {
+        force_x[p] = o[3 * p]; force_y[p] = o[3 * p + 1]; force_z[p] = o[3 * p + 2];
+    }
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// systems
+// ---------------------------------------------------------------------------------
+
+struct md_rl_ctx {
+    md_rl_params p{};
+    DeviceSystem s;
+    std::vector<float> eps, sigma, mass;
+    std::vector<int32_t> h_type;   // input order
+    std::vector<float> h_q;        // input order, plain charges
+    double qscale = 0.0;           // qs = q * qscale
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {};
+    struct StepEvents { hipEvent_t f0, f1, s0, s1; };
+    std::vector<StepEvents> tev;   // per timed step, reused batch to batch
+    int tev_used = 0;
+    double force_ms = 0.0, step_ms = 0.0;   // averages over the last synchronised batch
+    int timed_steps = 0;
+    bool have_particles = false;
+    float4 *d_tmp4 = nullptr;      // n: scratch for read-back of full-step velocities
+    md_rl_table *own_table = nullptr;
+};
+
+static void free_ctx_device(md_rl_ctx *c)
+{
+    DeviceSystem &s = c->s;
+    void *ptrs[] = {s.pos, s.pos2, s.vel, s.vel2, s.type, s.type2, s.orig, s.orig2, s.cell,
+                    s.perm, s.cell_start, s.cell_next, s.cell_fill, s.fix, s.f32, s.tab, s.etab,
+                    s.pair_ab, s.pair_ab_e, s.hk, s.mass, s.counters, s.red, c->d_tmp4};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &t : c->tev)
+        for (hipEvent_t e : {t.f0, t.f1, t.s0, t.s1})
+            if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" int md_rl_destroy(md_rl_ctx *c)
+{
+    if (!c) return MDRL_OK;
+    free_ctx_device(c);
+    md_rl_table_free(c->own_table);
+    delete c;
+    return MDRL_OK;
+}
+
+// Segment of r2 under the table's addressing, or -1 when r2 lies outside the table.
+static int table_segment(float r2, int off, int nseg)
+{
+    uint32_t u;
+    memcpy(&u, &r2, 4);
+    int seg = (int)((u >> 23) & 0xff) - off;
+    return (seg < 0 || seg >= nseg) ? -1 : seg;
+}
+
+extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
+{
+    if (!out || !p) { set_error("md_rl_create: NULL argument"); return MDRL_EINVAL; }
+    *out = nullptr;
+    if (p->n < 0 || p->rc <= 0 || p->ntypes < 1 || p->ntypes > 8 || !p->eps || !p->sigma || !p->mass ||
+        !(p->box[0] > 0 && p->box[1] > 0 && p->box[2] > 0) || !(p->dt > 0)) {
+        set_error("md_rl_create: invalid parameters (n=%d rc=%g ntypes=%d)", p->n, p->rc, p->ntypes);
+        return MDRL_EINVAL;
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    md_rl_ctx *c = new md_rl_ctx;
+    c->p = *p;
+    c->eps.assign(p->eps, p->eps + p->ntypes);
+    c->sigma.assign(p->sigma, p->sigma + p->ntypes);
+    c->mass.assign(p->mass, p->mass + p->ntypes);
+    c->p.eps = c->eps.data(); c->p.sigma = c->sigma.data(); c->p.mass = c->mass.data();
+    DeviceSystem &s = c->s;
+    Geometry &g = s.g;
+    g.n = p->n;
+    for (int d = 0; d < 3; ++d) {
+        int nc = p->ncell[d] > 0 ? p->ncell[d] : (int)floor((double)p->box[d] / (double)p->rc);
+        if (nc < 1) nc = 1;
+        if ((double)p->box[d] / nc < (double)p->rc) {
+            set_error("md_rl_create: cell edge %g < rc %g on axis %d", p->box[d] / nc, p->rc, d);
+            delete c;
+            return MDRL_ECELL;
+        }
+        g.nc[d] = nc;
+        g.box[d] = p->box[d];
+        g.inv_edge[d] = (float)((double)nc / (double)p->box[d]);
+    }
+    g.ncells = g.nc[0] * g.nc[1] * g.nc[2];
+    g.rc2 = p->rc * p->rc;
+    s.ntypes = p->ntypes;
+    s.flags = p->flags;
+    const md_rl_table *t = p->table;
+    if (!t) {
+        if ((rc = md_rl_table_load_default(&c->own_table)) != MDRL_OK) { delete c; return rc; }
+        t = c->own_table;
+    }
+    s.exp_offset = t->exp_offset;
+    s.nseg = t->nseg;
+    s.coul = p->coulomb_k != 0.0f && !(p->flags & MDRL_NO_COULOMB);
+    if (s.coul && !t->has[MDRL_TERM_3] && !(p->flags & MDRL_EVAL_ANALYTIC)) {
+        set_error("md_rl_create: Coulomb requested but the table has no r^-3 term");
+        delete c;
+        return MDRL_EINVAL;
+    }
+    int shi = table_segment(g.rc2, t->exp_offset, t->nseg);
+    if (shi < 0 && !(p->flags & MDRL_EVAL_ANALYTIC)) {
+        set_error("md_rl_create: rc^2 = %g lies outside the table range", (double)g.rc2);
+        delete c;
+        return MDRL_EINVAL;
+    }
+    s.win.seg_hi = shi < 0 ? 0 : shi;
+    s.win.seg_lo = s.win.seg_hi >= 8 ? s.win.seg_hi - 8 : 0;
+    s.win.stride = s.coul ? 6 : 4;
+    c->qscale = s.coul ? sqrt(10.0 * (double)p->coulomb_k * 4.0 * 3.14) : 0.0;
+
+    const int n = g.n, nt = p->ntypes;
+    int st = MDRL_OK;
+    if (!st) st = dalloc(&s.pos, n);
+    if (!st) st = dalloc(&s.pos2, n);
+    if (!st) st = dalloc(&s.vel, n);
+    if (!st) st = dalloc(&s.vel2, n);
+    if (!st) st = dalloc(&s.type, n);
+    if (!st) st = dalloc(&s.type2, n);
+    if (!st) st = dalloc(&s.orig, n);
+    if (!st) st = dalloc(&s.orig2, n);
+    if (!st) st = dalloc(&s.cell, n);
+    if (!st) st = dalloc(&s.perm, n);
+    if (!st) st = dalloc(&s.cell_start, (size_t)g.ncells + 1);
+    if (!st) st = dalloc(&s.cell_next, g.ncells);
+    if (!st) st = dalloc(&s.cell_fill, g.ncells);
+    if (!st) st = (p->flags & MDRL_ACC_FP32) ? dalloc(&s.f32, 4 * (size_t)n) : dalloc(&s.fix, 3 * (size_t)n);
+    if (!st) st = dalloc(&s.tab, (size_t)t->nent * 6);
+    if (!st) st = dalloc(&s.etab, (size_t)t->nent * 6);
+    if (!st) st = dalloc(&s.pair_ab, (size_t)nt * nt);
+    if (!st) st = dalloc(&s.pair_ab_e, (size_t)nt * nt);
+    if (!st) st = dalloc(&s.hk, nt);
+    if (!st) st = dalloc(&s.mass, nt);
+    if (!st) st = dalloc(&s.counters, 8);
+    if (!st) st = dalloc(&s.red, 16);
+    if (!st) st = dalloc(&c->d_tmp4, n);
+    if (st) { md_rl_destroy(c); return st; }
+
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { md_rl_destroy(c); return hip_fail(e, "hipStreamCreate"); }
+    for (auto &ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) { md_rl_destroy(c); return hip_fail(e, "hipEventCreate"); }
+
+    // type-pair multipliers: Lorentz-Berthelot, forces in kJ/mol/nm (10x folded in)
+    std::vector<float2> ab(nt * nt), abe(nt * nt);
+    const bool ana = p->flags & MDRL_EVAL_ANALYTIC;
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j) {
+            double sg = 0.5 * ((double)p->sigma[i] + (double)p->sigma[j]);
+            double ep = sqrt((double)p->eps[i] * (double)p->eps[j]);
+            double s6 = pow(sg, 6.0), s12 = s6 * s6;
+            ab[i * nt + j] = ana ? make_float2((float)(480.0 * ep * s12), (float)(240.0 * ep * s6))
+                                 : make_float2((float)(10.0 * ep * s12), (float)(10.0 * ep * s6));
+            abe[i * nt + j] = make_float2((float)(ep * s12), (float)(ep * s6));
+        }
+    std::vector<double> hk(nt), ms(nt);
+    for (int k = 0; k < nt; ++k) {
+        ms[k] = p->mass[k];
+        hk[k] = ldexp(1.0e-5 / (double)p->mass[k] * (0.5 * (double)p->dt), -32);
+    }
+    std::vector<float> tab = interleave(t, MDRL_TERM_14, MDRL_TERM_8, MDRL_TERM_3);
+    std::vector<float> etab = interleave(t, MDRL_TERM_12, MDRL_TERM_6, MDRL_TERM_1);
+    unsigned long long zeros[8] = {0};
+    double dz[16] = {0};
+#define UP(dst, src, bytes)                                                              \
+    if ((e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess) {          \
+        md_rl_destroy(c);                                                                 \
+        return hip_fail(e, "hipMemcpy");                                                  \
+    }
+    UP(s.pair_ab, ab.data(), sizeof(float2) * ab.size());
+    UP(s.pair_ab_e, abe.data(), sizeof(float2) * abe.size());
+    UP(s.hk, hk.data(), sizeof(double) * nt);
+    UP(s.mass, ms.data(), sizeof(double) * nt);
+    UP(s.tab, tab.data(), sizeof(float) * tab.size());
+    UP(s.etab, etab.data(), sizeof(float) * etab.size());
+    UP(s.counters, zeros, sizeof zeros);
+    UP(s.red, dz, sizeof dz);
+#undef UP
+    c->h_type.assign(n, 0);
+    c->h_q.assign(n, 0.0f);
+    *out = c;
+    return MDRL_OK;
+}
+
+// Uploads host particles (input order) and bins them; computes the forces.
+static int upload_and_bin(md_rl_ctx *c, const float *particle)
+{
+    DeviceSystem &s = c->s;
+    const int n = s.g.n;
+    std::vector<float4> pos(n), vel(n);
+    std::vector<int> orig(n);
+    for (int i = 0; i < n; ++i) {
+        float x[3];
+        for (int d = 0; d < 3; ++d) {
+            double v = particle[11 * (size_t)i + d], L = s.g.box[d];
+            if (!isfinite(v)) { set_error("md_rl_set_particles: non-finite position of atom %d", i); return MDRL_ENONFINITE; }
+            float f = (float)v;
+            if (!(f >= 0.0f && f < (float)L)) {
+                f = (float)(v - L * floor(v / L));
+                if (f >= (float)L) f = 0.0f;
+            }
+            x[d] = f;
+        }
+        pos[i] = make_float4(x[0], x[1], x[2], (float)((double)c->h_q[i] * c->qscale));
+        vel[i] = make_float4(particle[11 * (size_t)i + 3], particle[11 * (size_t)i + 4],
+                             particle[11 * (size_t)i + 5], 0.0f);
+        orig[i] = i;
+    }
+    hipError_t e;
+    if ((e = hipMemcpyAsync(s.pos, pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(s.vel, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(s.type, c->h_type.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(s.orig, orig.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
+    std::vector<int> zero(s.g.ncells, 0);
+    if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * s.g.ncells, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
+    int rc;
+    if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream))) return rc;
+    s.first_step = true;
+    c->have_particles = true;
+    if ((rc = launch_zero_forces(s, c->stream, true)) || (rc = launch_force(s, c->stream))) return rc;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    // input copies must stay alive until the async copies finished: synchronised above
+    return MDRL_OK;
+}
+
+// Current state back in input order (positions, stored velocities).
+static int download_state(md_rl_ctx *c, std::vector<float4> &pos, std::vector<float4> &vel,
+                          bool full_step_velocity)
+{
+    DeviceSystem &s = c->s;
+    const int n = s.g.n;
+    std::vector<float4> p(n), v(n);
+    std::vector<int> orig(n);
+    hipError_t e;
+    int rc;
+    if (full_step_velocity) {
+        if ((rc = launch_zero_red(s, c->stream)) || (rc = launch_kinetic(s, c->d_tmp4, c->stream))) return rc;
+    }
+    if ((e = hipMemcpyAsync(p.data(), s.pos, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(v.data(), full_step_velocity ? c->d_tmp4 : s.vel, sizeof(float4) * n,
+                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(orig.data(), s.orig, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return hip_fail(e, "read-back");
+    pos.resize(n);
+    vel.resize(n);
+    for (int k = 0; k < n; ++k) { pos[orig[k]] = p[k]; vel[orig[k]] = v[k]; }
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_types(md_rl_ctx *c, const int32_t *type, const float *charge, int n)
+{
+    if (!c || !type || n != c->s.g.n) { set_error("md_rl_set_types: bad argument"); return MDRL_EINVAL; }
+    for (int i = 0; i < n; ++i)
+        if (type[i] < 0 || type[i] >= c->s.ntypes) { set_error("md_rl_set_types: type %d of atom %d out of range", type[i], i); return MDRL_EINVAL; }
+    std::vector<float> p11;
+    if (c->have_particles) {
+        std::vector<float4> pos, vel;
+        int rc = download_state(c, pos, vel, false);
+        if (rc) return rc;
+        p11.assign((size_t)n * 11, 0.0f);
+        for (int i = 0; i < n; ++i) {
+            p11[11 * (size_t)i] = pos[i].x; p11[11 * (size_t)i + 1] = pos[i].y; p11[11 * (size_t)i + 2] = pos[i].z;
+            p11[11 * (size_t)i + 3] = vel[i].x; p11[11 * (size_t)i + 4] = vel[i].y; p11[11 * (size_t)i + 5] = vel[i].z;
+        }
+    }
+    c->h_type.assign(type, type + n);
+    if (charge) c->h_q.assign(charge, charge + n);
+    else c->h_q.assign(n, 0.0f);
+    if (c->have_particles) {
+        bool first = c->s.first_step;
+        int rc = upload_and_bin(c, p11.data());
+        c->s.first_step = first;
+        return rc;
+    }
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_particles(md_rl_ctx *c, const float *particle, int n)
+{
+    if (!c || !particle || n != c->s.g.n) { set_error("md_rl_set_particles: bad argument"); return MDRL_EINVAL; }
+    return upload_and_bin(c, particle);
+}
+
+static int check_flags(md_rl_ctx *c)
+{
+    unsigned long long ctr[8];
+    hipError_t e = hipMemcpy(ctr, c->s.counters, sizeof ctr, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    if (ctr[2] & MDRL_F_NONFINITE) { set_error("non-finite position, velocity or force"); return MDRL_ENONFINITE; }
+    return MDRL_OK;
+}
+
+// Events of the next timed step (created on first use, reused batch to batch).
+static md_rl_ctx::StepEvents *next_events(md_rl_ctx *c)
+{
+    if (c->tev_used == (int)c->tev.size()) {
+        md_rl_ctx::StepEvents t{};
+        if (hipEventCreate(&t.f0) != hipSuccess || hipEventCreate(&t.f1) != hipSuccess ||
+            hipEventCreate(&t.s0) != hipSuccess || hipEventCreate(&t.s1) != hipSuccess)
+            return nullptr;
+        c->tev.push_back(t);
+    }
+    return &c->tev[c->tev_used++];
+}
+
+// Averages the device times of the steps enqueued since the last synchronisation.
+static void collect_times(md_rl_ctx *c)
+{
+    if (!c->timing || c->tev_used == 0) return;
+    double fsum = 0.0, ssum = 0.0;
+    for (int k = 0; k < c->tev_used; ++k) {
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, c->tev[k].f0, c->tev[k].f1);
+        (void)hipEventElapsedTime(&b, c->tev[k].s0, c->tev[k].s1);
+        fsum += a;
+        ssum += b;
+    }
+    c->force_ms = fsum / c->tev_used;
+    c->step_ms = ssum / c->tev_used;
+    c->timed_steps = c->tev_used;
+    c->tev_used = 0;
+}
+
+extern "C" int md_rl_force_eval(md_rl_ctx *c)
+{
+    if (!c || !c->have_particles) { set_error("md_rl_force_eval: no particles set"); return MDRL_EINVAL; }
+    int rc;
+    md_rl_ctx::StepEvents *ev = c->timing ? next_events(c) : nullptr;
+    if (ev) { (void)hipEventRecord(ev->s0, c->stream); }
+    if ((rc = launch_zero_forces(c->s, c->stream, true))) return rc;
+    if (ev) (void)hipEventRecord(ev->f0, c->stream);
+    if ((rc = launch_force(c->s, c->stream))) return rc;
+    if (ev) { (void)hipEventRecord(ev->f1, c->stream); (void)hipEventRecord(ev->s1, c->stream); }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    collect_times(c);
+    return check_flags(c);
+}
+
+// One step: Motion_Update on the current forces, rebin (which clears the forces), forces at
+// the new positions.
+static int enqueue_step(md_rl_ctx *c, hipStream_t st)
+{
+    DeviceSystem &s = c->s;
+    int rc;
+    md_rl_ctx::StepEvents *ev = c->timing ? next_events(c) : nullptr;
+    if (ev) (void)hipEventRecord(ev->s0, st);
+    if ((rc = launch_update(s, c->p.dt, st))) return rc;
+    if ((rc = launch_rebin(s, st))) return rc;
+    if ((rc = launch_zero_forces(s, st, false))) return rc;
+    if (ev) (void)hipEventRecord(ev->f0, st);
+    if ((rc = launch_force(s, st))) return rc;
+    if (ev) { (void)hipEventRecord(ev->f1, st); (void)hipEventRecord(ev->s1, st); }
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
+{
+    if (!c || !c->have_particles || nsteps < 0) { set_error("md_rl_step: no particles set"); return MDRL_EINVAL; }
+    hipStream_t st = hip_stream ? (hipStream_t)hip_stream : c->stream;
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = enqueue_step(c, st);
+        if (rc) return rc;
+    }
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_sync(md_rl_ctx *c)
+{
+    if (!c) return MDRL_EINVAL;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    collect_times(c);
+    return check_flags(c);
+}
+
+extern "C" int md_rl_step(md_rl_ctx *c, int nsteps)
+{
+    int rc = md_rl_step_async(c, nsteps, nullptr);
+    if (rc) return rc;
+    return md_rl_sync(c);
+}
+
+extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)
+{
+    if (!c || !fix || n != c->s.g.n) { set_error("md_rl_get_forces_fixed: bad argument"); return MDRL_EINVAL; }
+    DeviceSystem &s = c->s;
+    std::vector<int> orig(n);
+    std::vector<long long> f(3 * (size_t)n);
+    hipError_t e;
+    if (s.flags & MDRL_ACC_FP32) {
+        std::vector<float> g(4 * (size_t)n);
+        if ((e = hipMemcpy(g.data(), s.f32, sizeof(float) * 4 * n, hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "hipMemcpy");
+        for (int k = 0; k < n; ++k)
+            for (int d = 0; d < 3; ++d) f[3 * (size_t)k + d] = llrint((double)g[4 * (size_t)k + d] * kFixScale);
+    } else if ((e = hipMemcpy(f.data(), s.fix, sizeof(long long) * 3 * n, hipMemcpyDeviceToHost)) != hipSuccess) {
+        return hip_fail(e, "hipMemcpy");
+    }
+    if ((e = hipMemcpy(orig.data(), s.orig, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy");
+    for (int k = 0; k < n; ++k)
+        for (int d = 0; d < 3; ++d) fix[3 * (size_t)orig[k] + d] = f[3 * (size_t)k + d];
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_get_particles(md_rl_ctx *c, float *particle, int n)
+{
+    if (!c || !particle || n != c->s.g.n) { set_error("md_rl_get_particles: bad argument"); return MDRL_EINVAL; }
+    std::vector<float4> pos, vel;
+    int rc = download_state(c, pos, vel, true);
+    if (rc) return rc;
+    std::vector<int64_t> fix(3 * (size_t)n);
+    if ((rc = md_rl_get_forces_fixed(c, fix.data(), n))) return rc;
+    for (int i = 0; i < n; ++i) {
+        float *p = particle + 11 * (size_t)i;
+        p[0] = pos[i].x; p[1] = pos[i].y; p[2] = pos[i].z;
+        p[3] = vel[i].x; p[4] = vel[i].y; p[5] = vel[i].z;
+        for (int d = 0; d < 3; ++d) p[6 + d] = (float)((double)fix[3 * (size_t)i + d] / kFixScale * 0.1);
+        p[9] = 0.0f;
+        p[10] = vel[i].w;
+    }
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9])
+{
+    if (!c || !c->have_particles) { set_error("md_rl_energy: no particles set"); return MDRL_EINVAL; }
+    DeviceSystem &s = c->s;
+    int rc;
+    if ((rc = launch_zero_red(s, c->stream)) || (rc = launch_kinetic(s, c->d_tmp4, c->stream)) ||
+        (rc = launch_energy(s, c->stream)))
+        return rc;
+    double red[16];
+    hipError_t e = hipMemcpyAsync(red, s.red, sizeof red, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    if (ke) *ke = red[0];
+    if (pe) *pe = red[1];
+    if (virial) for (int k = 0; k < 9; ++k) virial[k] = red[2 + k];
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *out)
+{
+    if (!c || !out) { set_error("md_rl_get_stats: bad argument"); return MDRL_EINVAL; }
+    unsigned long long ctr[8];
+    hipError_t e = hipMemcpy(ctr, c->s.counters, sizeof ctr, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    out->pair_evals = (int64_t)ctr[0];
+    out->candidates = (int64_t)ctr[1];
+    out->flags = (int)ctr[2];
+    out->max_cell_count = (int)ctr[3];
+    out->force_ms = c->force_ms;
+    out->step_ms = c->step_ms;
+    out->timed_steps = c->timed_steps;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_timing(md_rl_ctx *c, int on)
+{
+    if (!c) return MDRL_EINVAL;
+    c->timing = on != 0;
+    c->tev_used = 0;
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// the C model
+// ---------------------------------------------------------------------------------
+
+extern "C" void md_rl_cmodel_default(md_rl_cmodel_cfg *cfg)
+{
+    // R/C_Model_ForceEval/MD_Evaluation_Model.cpp:11-29, same promotions
+    float Nav = (float)6.022e23;
+    float Ar_weight = (float)39.95;
+    cfg->n = 500;
+    cfg->iterations = 100;
+    cfg->eps = (float)(1.995996 * 1.995996);
+    cfg->sigma = (float)(0.1675 * 2);
+    cfg->mass = Ar_weight / Nav / 1000.0f;
+    cfg->dt = (float)2E-15;
+    cfg->rc = 8.0f;
+    cfg->box[0] = cfg->box[1] = cfg->box[2] = 24.0f;
+    cfg->print = 1;
+}
+
+extern "C" int md_rl_cmodel_run(const md_rl_cmodel_cfg *cfg, float *particle, float *energies)
+{
+    if (!cfg || !particle || cfg->n <= 0 || cfg->iterations < 0) { set_error("md_rl_cmodel_run: bad argument"); return MDRL_EINVAL; }
+    int rc = check_device();
+    if (rc) return rc;
+    const int n = cfg->n;
+    std::vector<float> x(3 * (size_t)n), v(3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            x[d * (size_t)n + i] = particle[11 * (size_t)i + d];
+            v[d * (size_t)n + i] = particle[11 * (size_t)i + 3 + d];
+        }
+    CModelDev m{};
+    m.n = n;
+    const int iters = cfg->iterations;
+    int st = MDRL_OK;
+    if (!st) st = dalloc(&m.x, 3 * (size_t)n);
+    if (!st) st = dalloc(&m.v, 3 * (size_t)n);
+    if (!st) st = dalloc(&m.f, 3 * (size_t)n);
+    if (!st) st = dalloc(&m.e, n);
+    if (!st) st = dalloc(&m.ke, n);
+    if (!st) st = dalloc(&m.sums, 2 * (size_t)(iters > 0 ? iters : 1));
+    hipError_t e = hipSuccess;
+    if (!st && ((e = hipMemcpy(m.x, x.data(), sizeof(float) * x.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(m.v, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice)) != hipSuccess))
+        st = hip_fail(e, "hipMemcpy");
+    std::vector<float> sums(2 * (size_t)(iters > 0 ? iters : 1));
+    for (int it = 0; !st && it < iters; ++it) st = launch_cmodel_iteration(m, *cfg, it, 0);
+    if (!st && iters > 0 &&
+        (e = hipMemcpy(sums.data(), m.sums, sizeof(float) * 2 * iters, hipMemcpyDeviceToHost)) != hipSuccess)
+        st = hip_fail(e, "hipMemcpy");
+    std::vector<float> f(3 * (size_t)n), en(n), ke(n);
+    if (!st && ((e = hipMemcpy(x.data(), m.x, sizeof(float) * x.size(), hipMemcpyDeviceToHost)) != hipSuccess ||
+                (e = hipMemcpy(v.data(), m.v, sizeof(float) * v.size(), hipMemcpyDeviceToHost)) != hipSuccess ||
+                (e = hipMemcpy(f.data(), m.f, sizeof(float) * f.size(), hipMemcpyDeviceToHost)) != hipSuccess ||
+                (e = hipMemcpy(en.data(), m.e, sizeof(float) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+                (e = hipMemcpy(ke.data(), m.ke, sizeof(float) * n, hipMemcpyDeviceToHost)) != hipSuccess))
+        st = hip_fail(e, "hipMemcpy");
+    for (float *p : {m.x, m.v, m.f, m.e, m.ke, m.sums})
+        if (p) (void)hipFree(p);
+    if (st) return st;
+    for (int i = 0; i < n; ++i) {
+        float *p = particle + 11 * (size_t)i;
+        for (int d = 0; d < 3; ++d) {
+            p[d] = x[d * (size_t)n + i];
+            p[3 + d] = v[d * (size_t)n + i];
+            p[6 + d] = f[d * (size_t)n + i];
+        }
+        p[9] = en[i];
+        p[10] = ke[i];
+    }
+    for (int it = 0; it < iters; ++it) {
+        if (energies) { energies[2 * it] = sums[2 * it]; energies[2 * it + 1] = sums[2 * it + 1]; }
+        if (cfg->print)
+            printf("Iteration %d, System energy is %f, Kinetic Energy is %f\n", it,
+                   (double)sums[2 * it], (double)sums[2 * it + 1]);
+    }
+    return MDRL_OK;
+}
+
+// MD_Evaluation_Model(): reads the PDB (5 header lines, then "%*s x6 %f %f %f" per atom,
+// R/C_Model_ForceEval/MD_Evaluation_Model.cpp:38-55), velocities zero, runs the model.
+extern "C" int MD_Evaluation_Model(void)
+{
+    md_rl_cmodel_cfg cfg;
+    md_rl_cmodel_default(&cfg);
+    const char *path = getenv("MDRL_INPUT");
+    std::string def = "C:/Users/Ethan/Desktop/WorkingFolder/Ethan_RL_Pipeline_1st_Order_SingleFloat_18.0/"
+                      "SourceCode/MatlabScripts/ar_gas.pdb";
+    FILE *f = fopen(path && *path ? path : def.c_str(), "r");
+    if (!f) { set_error("MD_Evaluation_Model: cannot open %s", path && *path ? path : def.c_str()); return 0; }
+    std::vector<float> particle(11 * (size_t)cfg.n, 0.0f);
+    char line[1024];
+    for (int i = 0; i < 5; ++i)
+        if (!fgets(line, sizeof line, f)) line[0] = 0;
+    float tx = 0, ty = 0, tz = 0;   // as in the reference, a line that fails to parse repeats the last atom
+    for (int i = 0; i < cfg.n; ++i) {
+        if (!fgets(line, sizeof line, f)) line[0] = 0;
+        sscanf(line, "%*s %*s %*s %*s %*s %*s %f %f %f", &tx, &ty, &tz);
+        particle[11 * (size_t)i] = tx;
+        particle[11 * (size_t)i + 1] = ty;
+        particle[11 * (size_t)i + 2] = tz;
+    }
+    fclose(f);
+    return md_rl_cmodel_run(&cfg, particle.data(), nullptr) == MDRL_OK ? 1 : 0;
+}

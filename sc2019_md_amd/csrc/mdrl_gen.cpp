@@ -1,0 +1,223 @@
+// mdrl_gen.cpp -- seeded synthetic systems of the benchmark workloads.
+//
+// One generator serves the GPU runs and every CPU baseline so that both see identical
+// inputs.  Random numbers: SplitMix64 seeds xoshiro256**; uniforms are the top 53 bits;
+// normals are Box-Muller pairs.  Lattice site (i, j, k) sits at ((i, j, k) + 0.5) * a +
+// U(-jitter, +jitter)^3.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mdrl_internal.h"
+
+using mdrl::set_error;
+
+namespace {
+
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed)
+    {
+        uint64_t x = seed;
+        for (int k = 0; k < 4; ++k) s[k] = splitmix(x);
+    }
+    static uint64_t splitmix(uint64_t &x)
+    {
+        uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+    double normal()
+    {
+        if (has_spare) { has_spare = false; return spare; }
+        double u1 = 1.0 - uniform(), u2 = uniform();
+        double r = sqrt(-2.0 * log(u1)), th = 6.283185307179586 * u2;
+        spare = r * sin(th);
+        has_spare = true;
+        return r * cos(th);
+    }
+    uint64_t below(uint64_t n) { return (uint64_t)(uniform() * (double)n); }
+    bool has_spare = false;
+    double spare = 0.0;
+};
+
+constexpr double kB = 0.0083145;          // kJ/mol/K
+constexpr double kCoulomb = 1389.35;      // kJ A / (mol e^2)
+
+// Mixed-species recipe: types with fractions 0.50/0.25/0.15/0.10.
+const float kSig4[4] = {1.0f, 3.2f, 3.5f, 3.0f};
+const float kEps4[4] = {0.19f, 0.65f, 0.30f, 0.71f};
+// Masses are not part of the workload definition; H, C, N, O are used.
+const float kMass4[4] = {1.008f, 12.011f, 14.007f, 15.999f};
+
+void shuffle(std::vector<int> &v, Rng &r)
+{
+    for (size_t i = v.size(); i > 1; --i) {
+        size_t j = (size_t)r.below(i);
+        std::swap(v[i - 1], v[j]);
+    }
+}
+
+// n atoms on a jittered nx x ny x nz lattice of the box, sites chosen by permutation.
+void lattice(md_rl_system *s, int nx, int ny, int nz, double jitter, Rng &r)
+{
+    const int nsite = nx * ny * nz;
+    std::vector<int> sites(nsite);
+    for (int k = 0; k < nsite; ++k) sites[k] = k;
+    if (s->n < nsite) {
+        shuffle(sites, r);
+        sites.resize(s->n);
+        std::sort(sites.begin(), sites.end());
+    }
+    const double ax = s->box[0] / nx, ay = s->box[1] / ny, az = s->box[2] / nz;
+    for (int a = 0; a < s->n; ++a) {
+        int k = sites[a];
+        int i = k / (ny * nz), j = (k / nz) % ny, l = k % nz;
+        double p[3] = {(i + 0.5) * ax, (j + 0.5) * ay, (l + 0.5) * az};
+        for (int d = 0; d < 3; ++d) {
+            double x = p[d] + (2.0 * r.uniform() - 1.0) * jitter;
+            x -= s->box[d] * floor(x / s->box[d]);
+            float f = (float)x;
+            if (f >= s->box[d]) f = 0.0f;
+            s->particle[11 * a + d] = f;
+        }
+    }
+}
+
+void maxwell(md_rl_system *s, double T, Rng &r)
+{
+    double p[3] = {0, 0, 0}, mt = 0;
+    std::vector<double> v(3 * (size_t)s->n);
+    for (int a = 0; a < s->n; ++a) {
+        double m = s->mass[s->type[a]];
+        double sd = sqrt(kB * T / m) * 0.01;   // sqrt(kJ/mol/amu) = 1 nm/ps = 0.01 A/fs
+        for (int d = 0; d < 3; ++d) { v[3 * a + d] = r.normal() * sd; p[d] += m * v[3 * a + d]; }
+        mt += m;
+    }
+    for (int a = 0; a < s->n; ++a)
+        for (int d = 0; d < 3; ++d) s->particle[11 * a + 3 + d] = (float)(v[3 * a + d] - p[d] / mt);
+}
+
+void species4(md_rl_system *s, Rng &r)
+{
+    s->ntypes = 4;
+    for (int t = 0; t < 4; ++t) { s->sigma[t] = kSig4[t]; s->eps[t] = kEps4[t]; s->mass[t] = kMass4[t]; }
+    const int n = s->n;
+    int cnt[4];
+    cnt[1] = (int)lround(0.25 * n);
+    cnt[2] = (int)lround(0.15 * n);
+    cnt[3] = (int)lround(0.10 * n);
+    cnt[0] = n - cnt[1] - cnt[2] - cnt[3];
+    std::vector<int> ty;
+    ty.reserve(n);
+    for (int t = 0; t < 4; ++t) ty.insert(ty.end(), cnt[t], t);
+    shuffle(ty, r);
+    for (int a = 0; a < n; ++a) s->type[a] = ty[a];
+    double mean = 0;
+    std::vector<double> q(n);
+    for (int a = 0; a < n; ++a) { q[a] = 0.4 * r.normal(); mean += q[a]; }
+    mean /= n;
+    for (int a = 0; a < n; ++a) s->charge[a] = (float)(q[a] - mean);
+    s->coulomb_k = (float)kCoulomb;
+}
+
+int alloc(md_rl_system *s, int n)
+{
+    memset(s, 0, sizeof *s);
+    s->n = n;
+    s->particle = (float *)calloc((size_t)n * 11, sizeof(float));
+    s->type = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    s->charge = (float *)calloc((size_t)n, sizeof(float));
+    if (!s->particle || !s->type || !s->charge) {
+        free(s->particle); free(s->type); free(s->charge);
+        set_error("md_rl_make_system: out of host memory for %d atoms", n);
+        return MDRL_ENOMEM;
+    }
+    return MDRL_OK;
+}
+
+// Size of replica `index` of the batched set: log-uniform in [256, 16384].
+int replica_size(int index, uint64_t seed)
+{
+    Rng r(seed ^ 0x5bd1e995ull);
+    int n = 0;
+    for (int k = 0; k <= index; ++k) n = (int)lround(exp(log(256.0) + r.uniform() * (log(16384.0) - log(256.0))));
+    return n;
+}
+
+}  // namespace
+
+extern "C" int md_rl_make_system(md_rl_system *s, int which, int index, uint64_t seed)
+{
+    if (!s) { set_error("md_rl_make_system: NULL system"); return MDRL_EINVAL; }
+    int rc;
+    switch (which) {
+    case 1: {   // 512 argon atoms, 2x2x2 per cell of a 4^3 grid of 8.5 A cells
+        if ((rc = alloc(s, 512)) != MDRL_OK) return rc;
+        Rng r(seed);
+        s->box[0] = s->box[1] = s->box[2] = 34.0f;
+        s->rc = 8.5f; s->dt = 2.0f; s->ntypes = 1; s->coulomb_k = 0.0f;
+        s->sigma[0] = 3.4f; s->eps[0] = 0.996f; s->mass[0] = 39.948f;
+        lattice(s, 8, 8, 8, 0.3, r);
+        maxwell(s, 94.0, r);
+        return MDRL_OK;
+    }
+    case 2: {   // 92,224-atom LJ + Coulomb, 108.86 x 108.86 x 77.76 A
+        if ((rc = alloc(s, 92224)) != MDRL_OK) return rc;
+        Rng r(seed);
+        s->box[0] = 108.86f; s->box[1] = 108.86f; s->box[2] = 77.76f;
+        s->rc = 12.0f; s->dt = 1.0f;
+        lattice(s, 51, 51, 36, 0.4, r);
+        species4(s, r);
+        maxwell(s, 300.0, r);
+        return MDRL_OK;
+    }
+    case 3: {   // 1,048,576 atoms in a 219.0 A cube
+        if ((rc = alloc(s, 1048576)) != MDRL_OK) return rc;
+        Rng r(seed);
+        s->box[0] = s->box[1] = s->box[2] = 219.0f;
+        s->rc = 12.0f; s->dt = 1.0f;
+        lattice(s, 102, 102, 102, 0.4, r);
+        species4(s, r);
+        maxwell(s, 300.0, r);
+        return MDRL_OK;
+    }
+    case 4: {   // replica `index` of the 512-system batch: 0.1 atoms/A^3, rc = 10 A
+        if (index < 0) { set_error("md_rl_make_system: replica index < 0"); return MDRL_EINVAL; }
+        const int n = replica_size(index, seed);
+        if ((rc = alloc(s, n)) != MDRL_OK) return rc;
+        Rng r(Rng::splitmix(seed += 0x632be59bd9b4e019ull * (uint64_t)(index + 1)));
+        const float L = (float)cbrt(n / 0.1);
+        s->box[0] = s->box[1] = s->box[2] = L;
+        s->rc = 10.0f; s->dt = 1.0f;
+        const int m = (int)ceil(cbrt((double)n) - 1e-9);
+        lattice(s, m, m, m, 0.4, r);
+        species4(s, r);
+        maxwell(s, 300.0, r);
+        return MDRL_OK;
+    }
+    default:
+        set_error("md_rl_make_system: unknown workload %d", which);
+        return MDRL_EINVAL;
+    }
+}
+
+extern "C" void md_rl_free_system(md_rl_system *s)
+{
+    if (!s) return;
+    free(s->particle); free(s->type); free(s->charge);
+    s->particle = nullptr; s->type = nullptr; s->charge = nullptr;
+}

@@ -1,0 +1,77 @@
+"""The C-ABI library loads and exports every entry point include/mdrl.h declares; the
+host-only entry points work without a GPU; bad arguments are rejected (CPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sc2019_md_amd import mdrl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "mdrl.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n.startswith(("md_rl_", "MD_"))))
+
+
+def test_library_exports_every_declared_symbol():
+    L = mdrl.lib()
+    names = declared_functions()
+    assert "md_rl_create" in names and "MD_Evaluation_Model" in names and len(names) > 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_channel():
+    assert mdrl.lib().md_rl_version() == 1
+    n = mdrl.lib().md_rl_table_read_file(b"/nonexistent/c0_14.mif",
+                                        np.zeros(4, np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 4)
+    assert n == -3
+    assert b"cannot open" in mdrl.lib().md_rl_last_error()
+
+
+def test_default_tables_load():
+    t = mdrl.Table.load()
+    for term in range(6):
+        c0, c1 = t.get(term)
+        assert c0.shape == (3584,) and np.all(np.isfinite(c0)) and np.all(np.isfinite(c1))
+
+
+def test_table_load_rejects_bad_directory():
+    with pytest.raises(mdrl.MdrlError) as e:
+        mdrl.Table.load("/nonexistent")
+    assert e.value.code == -3
+
+
+def test_synthetic_systems_are_reproducible():
+    a = mdrl.SyntheticSystem(2, seed=5)
+    b = mdrl.SyntheticSystem(2, seed=5)
+    c = mdrl.SyntheticSystem(2, seed=6)
+    assert a.n == 92224 and np.array_equal(a.particle, b.particle)
+    assert not np.array_equal(a.particle, c.particle)
+    counts = np.bincount(a.type, minlength=4)
+    assert counts.tolist() == [46112, 23056, 13834, 9222]
+    assert abs(float(a.charge.astype(np.float64).sum())) < 1e-3
+    assert np.all(a.particle[:, :3] >= 0) and np.all(a.particle[:, :3] < a.box)
+
+
+def test_replica_sizes_log_uniform():
+    sizes = [mdrl.SyntheticSystem(4, k).n for k in range(64)]
+    assert min(sizes) >= 256 and max(sizes) <= 16384
+    s = mdrl.SyntheticSystem(4, 0)
+    assert s.box[0] == pytest.approx((s.n / 0.1) ** (1 / 3), rel=1e-6)
+
+
+def test_create_rejects_bad_parameters():
+    # validated before any device call
+    with pytest.raises(mdrl.MdrlError) as e:
+        mdrl.System(10, (10, 10, 10), -1.0, [1.0], [1.0], [1.0])
+    assert e.value.code == -1
+    with pytest.raises(mdrl.MdrlError) as e:
+        mdrl.System(10, (10, 10, 10), 3.0, [1.0] * 9, [1.0] * 9, [1.0] * 9)
+    assert e.value.code == -1

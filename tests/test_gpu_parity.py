@@ -1,0 +1,228 @@
+"""GPU parity: the HIP path through the C ABI against the oracle (MI355X only).
+
+Bars: bit-exact for int64 fixed-point forces, positions, velocities, pair counts and the
+golden pair records; FP32-atomics forces within 1e-5 relative L2 of the int64 forces
+(their order of summation is not fixed); the C model bit-exact in its printed energies.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sc2019_md_amd import mdrl
+
+pytestmark = pytest.mark.gpu
+
+from tests.test_oracle_golden import (  # noqa: E402
+    apoa1_tab6, cmodel_inputs, hex2f, parse_pdb_float32, parse_reference_output, pdb_text,
+    rl_from_synthetic)
+
+
+@pytest.fixture(scope="module")
+def table():
+    return mdrl.Table.load()
+
+
+@pytest.fixture(scope="module")
+def apoa1(table):
+    return {k: table.get(k) for k in range(6)}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------------------ pair batch
+def test_eval_pairs_golden_bit_exact(table):
+    """md_rl_eval_pairs (the OpenCL LJ kernel's interface) reproduces all 10,000 records."""
+    say("pairs: golden 10,000 records, parity+strict")
+    recs = parse_reference_output()
+    ref = hex2f(recs[:, 0:3].ravel()).reshape(-1, 3)
+    nbr = hex2f(recs[:, 3:6].ravel()).reshape(-1, 3)
+    out = mdrl.eval_pairs(table, ref, nbr, 196.0, mdrl.PAIR_PARITY | mdrl.PAIR_STRICT,
+                          ref_id=np.arange(10000) // 100, nbr_id=np.arange(10000) % 100)
+    want = hex2f(recs[:, 7:10].ravel()).reshape(-1, 3)
+    assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+    fs, fx = mdrl.eval_pairs_ex(table, ref, nbr, 196.0, mdrl.PAIR_PARITY | mdrl.PAIR_STRICT)
+    assert np.array_equal(fs.view(np.uint32), hex2f(recs[:, 6]).view(np.uint32))
+
+
+def test_eval_pairs_fast_mode_close(table):
+    """Fused arithmetic differs from the golden records by rounding only."""
+    recs = parse_reference_output()
+    ref = hex2f(recs[:, 0:3].ravel()).reshape(-1, 3)
+    nbr = hex2f(recs[:, 3:6].ravel()).reshape(-1, 3)
+    fast = mdrl.eval_pairs(table, ref, nbr, 196.0, mdrl.PAIR_STRICT)
+    want = hex2f(recs[:, 7:10].ravel()).reshape(-1, 3)
+    assert np.linalg.norm(fast - want) / np.linalg.norm(want) < 1e-5
+
+
+def test_eval_pairs_edge_cases(table):
+    """r2 == 0 and r2 > rc2 give 0; r2 == rc2 is accepted unless strict; empty batch."""
+    ref = np.array([[1, 1, 1], [0, 0, 0], [0, 0, 0], [0, 0, 0]], np.float32)
+    nbr = np.array([[1, 1, 1], [12, 0, 0], [12.5, 0, 0], [3, 0, 0]], np.float32)
+    out = mdrl.eval_pairs(table, ref, nbr, 144.0)
+    assert np.all(out[0] == 0) and out[1, 0] != 0 and np.all(out[2] == 0) and out[3, 0] != 0
+    out = mdrl.eval_pairs(table, ref, nbr, 144.0, mdrl.PAIR_STRICT)
+    assert out[1, 0] == 0
+    assert mdrl.eval_pairs(table, ref[:0], nbr[:0], 144.0).shape == (0, 3)
+
+
+# ------------------------------------------------------------------------ C model
+@pytest.mark.parametrize("case", ["uniform", "corner_cluster"])
+def test_cmodel_gpu_matches_oracle(case):
+    """The C model on the GPU (md_rl_cmodel_run) against its CPU restatement, 100 iterations."""
+    say("cmodel:", case)
+    text = pdb_text(cmodel_inputs()[case])
+    p = np.zeros((500, 11), np.float32)
+    p[:, :3] = parse_pdb_float32(text)
+    params = O.cmodel_default_params()
+    want, we, _ = O.cmodel_run(p, params)
+    got, ge = mdrl.cmodel_run(p)
+    assert np.array_equal(ge.view(np.uint32), we.view(np.uint32))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+# ------------------------------------------------------------------------ half-shell forces
+def gpu_system(s, flags=0):
+    return mdrl.System.from_synthetic(s, flags=flags)
+
+
+CASES = [(1, 0), (4, 0), (4, 3), (4, 11), (4, 40), (2, 0)]
+
+
+@pytest.mark.parametrize("which,index", CASES)
+@pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])
+def test_halfshell_int64_bit_exact(apoa1, which, index, mode):
+    """int64 forces of the HIP half-shell kernel equal the oracle's bit for bit."""
+    s = mdrl.SyntheticSystem(which, index)
+    say("halfshell int64: workload %d replica %d n=%d mode=%s" % (which, index, s.n, mode))
+    flags = {"fast": 0, "parity": mdrl.EVAL_PARITY, "analytic": mdrl.EVAL_ANALYTIC}[mode]
+    omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
+    g = gpu_system(s, flags)
+    got = g.forces_fixed()
+    st = g.stats()
+    g.close()
+    rl = rl_from_synthetic(s, apoa1, mode=omode)
+    want, npairs = rl.forces_fix()
+    assert st.pair_evals == npairs
+    bad = np.nonzero(np.any(got != want, axis=1))[0]
+    assert bad.size == 0, "atoms differ: %s" % bad[:10]
+
+
+@pytest.mark.parametrize("which,index", [(1, 0), (4, 3), (2, 0)])
+def test_halfshell_fp32_atomics_close(which, index):
+    s = mdrl.SyntheticSystem(which, index)
+    say("halfshell fp32: workload %d replica %d" % (which, index))
+    a = gpu_system(s).forces_fixed().astype(np.float64)
+    b = gpu_system(s, mdrl.ACC_FP32).forces_fixed().astype(np.float64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-5
+
+
+def test_small_boxes_all_images(apoa1):
+    """Replicas with one or two cells per axis: every periodic image within rc counted once."""
+    for index in range(64):
+        s = mdrl.SyntheticSystem(4, index)
+        if s.box[0] >= 30.0:
+            continue
+        g = gpu_system(s)
+        st = g.stats()
+        g.close()
+        _, _, _, nb = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                    use_cells=False)
+        assert st.pair_evals == nb, "replica %d (L=%.2f)" % (index, s.box[0])
+
+
+# ------------------------------------------------------------------------ steps
+@pytest.mark.parametrize("which,index,nsteps", [(1, 0, 20), (4, 5, 10), (2, 0, 3)])
+def test_steps_bit_exact(apoa1, which, index, nsteps):
+    """Velocity-Verlet steps (update + rebin + forces) equal the oracle's step by step."""
+    s = mdrl.SyntheticSystem(which, index)
+    say("steps: workload %d replica %d n=%d steps=%d" % (which, index, s.n, nsteps))
+    g = gpu_system(s)
+    rl = rl_from_synthetic(s, apoa1)
+    hk = O.half_kick(s.mass, s.dt)
+    pos4 = rl.pos4.copy()
+    vel = s.particle[:, 3:6].copy()
+    fix, _ = rl.forces_fix()
+    for k in range(nsteps):
+        pos4, vel, _ = O.motion_update(pos4, vel, fix, s.type, hk, s.mass, s.dt, s.box, k == 0)
+        rl.pos4[:] = pos4
+        fix, _ = rl.forces_fix()
+    g.step(nsteps)
+    got = g.particles()
+    gfix = g.forces_fixed()
+    g.close()
+    assert np.array_equal(got[:, :3].view(np.uint32), pos4[:, :3].view(np.uint32))
+    assert np.array_equal(gfix, fix)
+    # full-step velocity = vh + dv(F)
+    dv = (fix.astype(np.float64) * hk[s.type][:, None]).astype(np.float32)
+    vfull = (vel + dv).astype(np.float32)
+    assert np.array_equal(got[:, 3:6].view(np.uint32), vfull.view(np.uint32))
+
+
+def test_kick_drift_mode(apoa1):
+    s = mdrl.SyntheticSystem(1)
+    g = gpu_system(s, mdrl.KICK_DRIFT)
+    rl = rl_from_synthetic(s, apoa1)
+    hk = O.half_kick(s.mass, s.dt)
+    pos4, vel = rl.pos4.copy(), s.particle[:, 3:6].copy()
+    for k in range(5):
+        fix, _ = rl.forces_fix()
+        pos4, vel, _ = O.motion_update(pos4, vel, fix, s.type, hk, s.mass, s.dt, s.box, False,
+                                       kick_drift=True)
+        rl.pos4[:] = pos4
+    g.step(5)
+    got = g.particles()
+    g.close()
+    assert np.array_equal(got[:, :3].view(np.uint32), pos4[:, :3].view(np.uint32))
+
+
+def test_deterministic_repeats():
+    """int64 accumulation: two runs of the same steps give identical bits."""
+    s = mdrl.SyntheticSystem(2)
+    out = []
+    for _ in range(2):
+        g = gpu_system(s)
+        g.step(5)
+        out.append(g.particles())
+        g.close()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+
+
+def test_energy_vs_fp64(apoa1):
+    """FP64-reduced table energy vs the FP64 analytic energy; kinetic energy closed form."""
+    s = mdrl.SyntheticSystem(1)
+    g = gpu_system(s)
+    pe, ke, vir = g.energy()
+    g.close()
+    _, e64, _, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc)
+    assert pe == pytest.approx(e64, rel=1e-4)
+    v = s.particle[:, 3:6].astype(np.float64)
+    ke_want = 0.5 * float(s.mass[0]) * float((v * v).sum()) * 1e4
+    assert ke == pytest.approx(ke_want, rel=1e-12)
+
+
+def test_nve_energy_drift():
+    """Argon, 2,000 steps of 2 fs: total energy stays within 0.5% of |PE|."""
+    s = mdrl.SyntheticSystem(1)
+    g = gpu_system(s)
+    pe0, ke0, _ = g.energy()
+    g.step(2000)
+    pe1, ke1, _ = g.energy()
+    g.close()
+    assert abs((pe1 + ke1) - (pe0 + ke0)) < 5e-3 * abs(pe0)
+
+
+def test_invalid_inputs_raise():
+    s = mdrl.SyntheticSystem(1)
+    with pytest.raises(mdrl.MdrlError):
+        mdrl.System(s.n, s.box, 20.0, s.eps, s.sigma, s.mass)   # rc^2 = 400 beyond the table
+    g = gpu_system(s)
+    bad = s.particle.copy()
+    bad[3, 0] = np.nan
+    with pytest.raises(mdrl.MdrlError):
+        g.set_particles(bad)
+    g.close()

@@ -1,0 +1,296 @@
+"""The oracle pinned against the reference's own golden vectors and known answers (CPU).
+
+Fixtures (tests/golden/PROVENANCE.json lists source paths and SHA-256):
+  VERIFICATION_REFERENCE_OUTPUT.txt.gz   10,000 pair records from
+      MatlabScripts/Simulation_Verification_Raw_ApoA1_LJ_no_smooth.m:113-200
+  VERIFICATION_PARTICLE_PAIR_INPUT.sha256  the half-shell pair stream of
+      MatlabScripts/Simulation_Verification_LJArgon.m:518-572
+  input_positions_ljargon_20000_box_58_49_49.txt.gz   its input
+"""
+import gzip
+import hashlib
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sc2019_md_amd import mdrl
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TABLES = os.path.join(os.path.dirname(GOLD), "..", "sc2019_md_amd", "tables", "apoa1")
+
+
+def hex2f(h):
+    return np.array([int(x, 16) for x in h], dtype=np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def apoa1():
+    t = mdrl.Table.load()
+    return {k: t.get(k) for k in range(6)}
+
+
+def parse_reference_output():
+    with gzip.open(os.path.join(GOLD, "VERIFICATION_REFERENCE_OUTPUT.txt.gz"), "rt") as f:
+        lines = f.read().splitlines()[2:]
+    recs = []
+    for a, b in zip(lines[0::2], lines[1::2]):
+        head, forces = a.split(": ")
+        pa, pb = head[1:-1].split("),(")
+        dist = b.split(": ")[1]
+        recs.append(pa.split(",") + pb.split(",") + forces.split("\t") + dist.split(", "))
+    return np.array(recs)
+
+
+def test_reference_output_bit_exact(apoa1):
+    """All 10,000 records: F, Fx, Fy, Fz, r2, dx, dy, dz bit-identical."""
+    recs = parse_reference_output()
+    assert recs.shape == (10000, 14)
+    ref = hex2f(recs[:, 0:3].ravel()).reshape(-1, 3)
+    nbr = hex2f(recs[:, 3:6].ravel()).reshape(-1, 3)
+    want = recs[:, 6:14]
+    c0_14, c1_14 = apoa1[mdrl.TERM_14]
+    c0_8, c1_8 = apoa1[mdrl.TERM_8]
+    out = O.pair_golden(c0_14, c1_14, c0_8, c1_8, ref, nbr, 196.0, strict=True)
+    got = np.array([["%08X" % v for v in row] for row in out.view(np.uint32)])
+    mism = np.nonzero(np.any(got != want, axis=1))[0]
+    assert mism.size == 0, "records differ: %s" % mism[:10]
+    # 9,894 of the records lie inside the cutoff
+    assert int(np.count_nonzero(out[:, 0])) == 9894
+
+
+def test_reference_output_positions_are_apoa1_rows():
+    recs = parse_reference_output()
+    rows = np.loadtxt(os.path.join(GOLD, "input_positions_ApoA1_first100.txt"))
+    ref = hex2f(recs[:, 0:3].ravel()).reshape(100, 100, 3)
+    assert np.array_equal(ref[:, 0, :], rows.astype(np.float32))
+
+
+def test_fused_rounding_breaks_parity(apoa1):
+    """Planted mistake on the CPU: with fused c1*r2 + c0 most records no longer match, so the
+    bit-exact test above really pins the rounding order."""
+    recs = parse_reference_output()
+    ref = hex2f(recs[:, 0:3].ravel()).reshape(-1, 3)
+    nbr = hex2f(recs[:, 3:6].ravel()).reshape(-1, 3)
+    c0_14, c1_14 = apoa1[mdrl.TERM_14]
+    c0_8, c1_8 = apoa1[mdrl.TERM_8]
+    out = O.pair_golden(c0_14, c1_14, c0_8, c1_8, ref, nbr, 196.0)
+    r2 = out[:, 4].astype(np.float64)
+    idx = np.array([O.lut_index(x) for x in out[:, 4]])
+    t14 = (c1_14[idx].astype(np.float64) * r2 + c0_14[idx]).astype(np.float32)
+    t8 = (c1_8[idx].astype(np.float64) * r2 + c0_8[idx]).astype(np.float32)
+    fused = (t14 - t8) * (out[:, 0] != 0)
+    assert np.count_nonzero(fused.view(np.uint32) != out[:, 0].view(np.uint32)) > 1000
+
+
+def test_pair_stream_reproduces_reference_file():
+    """Binning (1-based ceil, stable order) + 8-filter half-shell stream == the 3.8 MB file."""
+    with gzip.open(os.path.join(GOLD, "input_positions_ljargon_20000_box_58_49_49.txt.gz"), "rt") as f:
+        raw = np.array([[float(x) for x in l.split()[1:4]] for l in f if l.strip()])
+    assert raw.shape == (20000, 3)
+    text = O.pair_stream_text(raw, 8.5, (7, 6, 6), (2, 2, 2), 100)
+    want_sha, want_lines = open(os.path.join(GOLD, "VERIFICATION_PARTICLE_PAIR_INPUT.sha256")).read().split()
+    got_lines = text.split(b"\n")
+    assert len(got_lines) == int(want_lines)
+    with gzip.open(os.path.join(GOLD, "VERIFICATION_PARTICLE_PAIR_INPUT.head.txt.gz"), "rb") as f:
+        head = f.read().split(b"\n")
+    for k, (g, w) in enumerate(zip(got_lines[:720], head[:720])):
+        assert g == w, "line %d differs" % k
+    assert hashlib.sha256(text).hexdigest() == want_sha
+
+
+def test_table_formats_agree():
+    """Intel-HEX (OpenCL_HDL_Lib/HDL_Source/c0_14.hex) and .mif copies hold the same words."""
+    a = mdrl.read_table_file(os.path.join(GOLD, "c0_14.hex"))
+    b = mdrl.read_table_file(os.path.join(mdrl.default_table_dir(), "c0_14.mif"))
+    assert a.shape == (3584,) and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("term,k,scale,tol", [
+    (mdrl.TERM_14, 14, 48.0, 1.1e-4), (mdrl.TERM_8, 8, 24.0, 4e-5),
+    (mdrl.TERM_3, 3, 1.0 / (4 * 3.14), 1e-5), (mdrl.TERM_12, 12, 4.0, 1e-4),
+    (mdrl.TERM_6, 6, 4.0, 3e-5), (mdrl.TERM_1, 1, 1.0 / (4 * 3.14), 5e-6)])
+def test_table_matches_closed_form(apoa1, term, k, scale, tol):
+    """Indexing by {exp - 121, mantissa[22:15]} with t = c1 r2 + c0 reproduces scale * r^-k
+    over r2 in [2^-6, 2^8): pins the offset, the bin bits and the c0/c1 roles."""
+    rng = np.random.default_rng(1)
+    r2 = np.float32(2.0) ** rng.uniform(-6, 8, 20000).astype(np.float32)
+    c0, c1 = apoa1[term]
+    idx = np.array([O.lut_index(x) for x in r2])
+    t = c1[idx].astype(np.float64) * r2 + c0[idx]
+    exact = scale * r2.astype(np.float64) ** (-k / 2.0)
+    rel = np.abs(t / exact - 1.0)
+    assert rel.max() < tol
+
+
+def test_lut_index_out_of_range_is_segment_zero():
+    # r2 below 2^-6 and above 2^8 fall back to segment 0 with the mantissa bin (HDL rule)
+    for r2 in (2.0 ** -7 * 1.5, 300.0):
+        u = np.array([r2], np.float32).view(np.uint32)[0]
+        assert O.lut_index(r2) == (u >> 15) & 0xFF
+    assert O.lut_index(1.0) == 6 * 256
+    assert O.lut_index(144.0) == 13 * 256 + ((np.array([144.0], np.float32).view(np.uint32)[0] >> 15) & 0xFF)
+
+
+# ------------------------------------------------------------------------------ C model
+def pdb_text(xyz):
+    lines = ["REMARK    generated for the C-model parity check"] * 5
+    for i, (x, y, z) in enumerate(xyz):
+        lines.append("ATOM  %5d  AR   AR A   1    %8.3f%8.3f%8.3f  1.00  0.00" % (i + 1, x, y, z))
+    return "\n".join(lines) + "\n"
+
+
+def parse_pdb_float32(text):
+    """The reference's sscanf("%f") reading, via libc strtof."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.strtof.restype = ctypes.c_float
+    libc.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    out = []
+    for l in text.splitlines()[5:]:
+        tok = l.split()
+        out.append([libc.strtof(t.encode(), None) for t in tok[6:9]])
+    return np.array(out, np.float32)
+
+
+def cmodel_inputs():
+    rng = np.random.default_rng(7)
+    uniform = rng.uniform(0, 24, (500, 3))
+    g = np.arange(8) * 0.40
+    cl = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)[:500] - 1.4
+    cluster = np.mod(cl + rng.uniform(-0.02, 0.02, cl.shape), 24.0)
+    return {"uniform": uniform, "corner_cluster": cluster}
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_BIN), reason="oracle/_ref not built (needs the reference tree)")
+@pytest.mark.parametrize("case", ["uniform", "corner_cluster"])
+def test_cmodel_matches_reference_binary(case):
+    """The restated C model prints exactly what the reference C model, compiled from
+    Range_Limited/C_Model_ForceEval/MD_Evaluation_Model.cpp, prints for the same PDB."""
+    text = pdb_text(cmodel_inputs()[case])
+    with tempfile.TemporaryDirectory() as d:
+        want = O.run_reference_cmodel(text, d)
+    xyz = parse_pdb_float32(text)
+    p = np.zeros((500, 11), np.float32)
+    p[:, :3] = xyz
+    params = O.cmodel_default_params()
+    _, _, lines = O.cmodel_run(p, params, with_lines=True)
+    assert lines == want
+    assert len(want.splitlines()) == 100
+
+
+def test_cmodel_two_atom_closed_form():
+    """Two atoms on the x axis: F = 48 eps s^12 r^-13 - 24 eps s^6 r^-7 (C-model units)."""
+    params = O.cmodel_default_params()
+    params.iterations = 1
+    p = np.zeros((2, 11), np.float32)
+    p[0, :3] = [10.0, 12.0, 12.0]
+    p[1, :3] = [10.45, 12.0, 12.0]
+    _, e, _ = O.cmodel_run(p, params)
+    cnt, f = O.cmodel_force_sample(p, params, 2)
+    assert cnt == 2
+    r = float(np.float32(10.45) - np.float32(10.0))
+    eps, s = float(params.eps), float(params.sigma)
+    fx = -(48 * eps * s ** 12 / r ** 13 - 24 * eps * s ** 6 / r ** 7)
+    assert f[0, 0] == pytest.approx(fx, rel=1e-5)
+    assert f[1, 0] == pytest.approx(-fx, rel=1e-5)
+    en = 4 * eps * ((s / r) ** 12 - (s / r) ** 6)
+    assert e[0, 0] == pytest.approx(en, rel=1e-5)
+
+
+# ------------------------------------------------------------------------------ half-shell model
+def apoa1_tab6(apoa1):
+    return O.interleave([*apoa1[mdrl.TERM_14], *apoa1[mdrl.TERM_8], *apoa1[mdrl.TERM_3]])
+
+
+def rl_from_synthetic(s, apoa1, mode=0):
+    ncell = [max(1, int(math.floor(float(s.box[d]) / s.rc))) for d in range(3)]
+    return O.RL(s.particle[:, :3], s.box, ncell, s.rc, s.type, s.eps, s.sigma, apoa1_tab6(apoa1),
+                charge=s.charge if s.coulomb_k else None, coulomb_k=s.coulomb_k, mode=mode)
+
+
+def test_half_shell_newton_third_law_exact(apoa1):
+    s = mdrl.SyntheticSystem(1)
+    rl = rl_from_synthetic(s, apoa1)
+    fix, npairs = rl.forces_fix()
+    assert np.all(fix.sum(axis=0) == 0)
+    assert 6000 < npairs < 9000
+
+
+def test_half_shell_pairs_equal_brute_force(apoa1):
+    """Every unordered in-cutoff pair (all images) is evaluated exactly once."""
+    s = mdrl.SyntheticSystem(1)
+    rl = rl_from_synthetic(s, apoa1)
+    _, npairs = rl.forces_fix()
+    _, _, _, nbrute = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                    use_cells=False)
+    assert npairs == nbrute
+
+
+@pytest.mark.parametrize("which,index", [(1, 0), (4, 3), (4, 11)])
+def test_half_shell_vs_fp64(apoa1, which, index):
+    """Kernel arithmetic vs FP64 of the same table model (error (i)), and the table model vs
+    the FP64 analytic oracle (error (ii)); analytic FP32 mode vs FP64 analytic."""
+    s = mdrl.SyntheticSystem(which, index)
+    rl = rl_from_synthetic(s, apoa1)
+    fix, _ = rl.forces_fix()
+    f = fix.astype(np.float64) / 2 ** 32 * 0.1
+    ft, _ = rl.forces_table_fp64()
+    fd, _, _, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                charge=s.charge if s.coulomb_k else None, ke=s.coulomb_k)
+    err_i = np.linalg.norm(f - ft) / np.linalg.norm(ft)
+    err_ii = np.linalg.norm(ft - fd) / np.linalg.norm(fd)
+    assert err_i < 1e-5
+    assert err_ii < 1e-4
+    rla = rl_from_synthetic(s, apoa1, mode=O.MODE_ANALYTIC)
+    fa, _ = rla.forces_fix()
+    err_a = np.linalg.norm(fa.astype(np.float64) / 2 ** 32 * 0.1 - fd) / np.linalg.norm(fd)
+    assert err_a < 1e-5
+
+
+def test_two_atom_table_force(apoa1):
+    """One pair along x across the periodic boundary: F = A*t14 - B*t8 of the table."""
+    box = np.array([40.0, 40.0, 40.0], np.float32)
+    xyz = np.array([[0.5, 20.0, 20.0], [36.5, 20.0, 20.0]], np.float32)   # r = 4 across x = 0
+    rl = O.RL(xyz, box, (4, 4, 4), 10.0, [0, 0], [0.996], [3.4], apoa1_tab6(apoa1))
+    fix, npairs = rl.forces_fix()
+    assert npairs == 1
+    f = fix.astype(np.float64) / 2 ** 32 * 0.1
+    r = 4.0
+    exact = 48 * 0.996 * 3.4 ** 12 / r ** 13 - 24 * 0.996 * 3.4 ** 6 / r ** 7
+    assert f[0, 0] == pytest.approx(exact, rel=2e-4)
+    assert f[1, 0] == -f[0, 0]
+
+
+def test_fix_rounding_and_saturation():
+    assert O.fix(1.0) == 2 ** 32
+    assert O.fix(-0.25) == -(2 ** 30)
+    assert O.fix(2.0 ** -33) == 0          # ties to even
+    assert O.fix(3 * 2.0 ** -33) == 2      # 1.5 -> 2
+    assert O.fix(1e30) == 2 ** 62
+    assert O.fix(-1e30) == -(2 ** 62)
+
+
+def test_motion_update_closed_forms():
+    """Free flight: x' = x + v dt.  Constant force: kick-drift and velocity Verlet both give
+    v_n = v_0 + n a dt; VV positions follow x0 + v0 t + a t^2 / 2."""
+    box = np.array([100, 100, 100], np.float32)
+    pos = np.array([[10, 20, 30, 0]], np.float32)
+    vel = np.array([[0.01, -0.02, 0.005]], np.float32)
+    hk = O.half_kick([40.0], 2.0)
+    p1, v1, _ = O.motion_update(pos, vel, np.zeros((1, 3), np.int64), [0], hk, [40.0], 2.0, box, True)
+    assert np.array_equal(v1, vel)
+    assert np.array_equal(p1[0, :3], (pos[0, :3] + vel[0] * np.float32(2.0)).astype(np.float32))
+    F = 2.0 ** 32 * 10.0                   # 10 kJ/mol/nm
+    fix = np.array([[int(F), 0, 0]], np.int64)
+    a = 10.0 * 1e-5 / 40.0                 # A/fs^2
+    p, v = pos.copy(), vel.copy()
+    for k in range(50):
+        p, v, _ = O.motion_update(p, v, fix, [0], hk, [40.0], 2.0, box, k == 0)
+    # vh after 50 steps = v0 + (50 - 0.5) * a * dt (half-step velocity)
+    assert float(v[0, 0]) == pytest.approx(0.01 + 49.5 * a * 2.0, rel=1e-5)
+    x_half = 10 + sum((0.01 + (k + 0.5) * a * 2.0) * 2.0 for k in range(50))
+    assert float(p[0, 0]) == pytest.approx(x_half, rel=1e-6)
