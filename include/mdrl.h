@@ -126,7 +126,7 @@ typedef struct md_rl_params {
     float rc;           /* cutoff (A) */
     int ncell[3];       /* cells per axis; 0 = floor(box/rc). Cell edge must be >= rc */
     float dt;           /* time step (fs) */
-    int ntypes;         /* atom types (1..64) */
+    int ntypes;         /* atom types (1..8) */
     const float *eps;   /* ntypes: LJ epsilon (kJ/mol), Lorentz-Berthelot mixing */
     const float *sigma; /* ntypes: LJ sigma (A) */
     const float *mass;  /* ntypes: mass (amu) */
@@ -165,6 +165,11 @@ int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream);
 int md_rl_sync(md_rl_ctx *c);
 /* Copies back positions, velocities (full step), forces and per-atom kinetic energy. */
 int md_rl_get_particles(md_rl_ctx *c, float *particle, int n);
+/* Capped steepest descent on the current positions (velocities untouched): each atom
+ * moves by F * min(gamma, max_disp / |F|) per iteration (F in kJ/mol/nm, gamma in
+ * A per kJ/mol/nm, max_disp in A), then rebin and forces.  Pre-relaxes jittered lattices
+ * whose overlaps would otherwise blow up the first steps. */
+int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, float gamma);
 /* Raw int64 fixed-point forces (2^-32 kJ/mol/nm units), n x 3, input order. */
 int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n);
 /* FP64 energy of the current positions from the c*_{12,6,1} tables (potential),

@@ -709,7 +709,27 @@ static inline float wrap1(float x, float L)
 {
     if (x < 0.0f) x = x + L;
     if (x >= L) x = x - L;
+    if (!(x >= 0.0f && x < L)) {
+        x = x - L * floorf(x / L);
+        if (!(x >= 0.0f && x < L)) x = 0.0f;
+    }
     return x;
+}
+
+void or_minimize_update(int n, float *pos4, const int64_t *fix, double gamma, double dmax,
+                        const float box[3])
+{
+    for (int i = 0; i < n; ++i) {
+        double f[3];
+        for (int d = 0; d < 3; ++d) f[d] = (double)fix[3 * i + d] * (1.0 / 4294967296.0);
+        double fm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        double sc = (fm * gamma > dmax) ? dmax / fm : gamma;
+        for (int d = 0; d < 3; ++d) {
+            float step = (float)(f[d] * sc);
+            float x = pos4[4 * i + d] + step;
+            pos4[4 * i + d] = wrap1(x, box[d]);
+        }
+    }
 }
 
 void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,

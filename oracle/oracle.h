@@ -158,6 +158,11 @@ void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,
                       const int *type, const double *hk, const double *mass, float dt,
                       const float box[3], int first, int kick_drift, double *ke_out);
 
+/* Capped steepest descent step (libmdrl's untimed pre-relaxation):
+ * F = fix * 2^-32, s = min(gamma, dmax / |F|), x' = wrap(x + (float)(F * s)). */
+void or_minimize_update(int n, float *pos4, const int64_t *fix, double gamma, double dmax,
+                        const float box[3]);
+
 #ifdef __cplusplus
 }
 #endif

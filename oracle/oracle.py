@@ -76,6 +76,8 @@ def lib():
         L.or_motion_update.argtypes = [ctypes.c_int, F, F, I64, I, D, D, ctypes.c_float, F,
                                        ctypes.c_int, ctypes.c_int, D]
         L.or_motion_update.restype = None
+        L.or_minimize_update.argtypes = [ctypes.c_int, F, I64, ctypes.c_double, ctypes.c_double, F]
+        L.or_minimize_update.restype = None
         _lib = L
     return _lib
 
@@ -288,3 +290,12 @@ def run_reference_cmodel(pdb_text, workdir):
     if r.returncode != 0:
         raise RuntimeError("reference C model failed: %s" % r.stderr)
     return r.stdout
+
+
+def minimize_update(pos4, fixf, gamma, dmax, box):
+    pos4 = np.ascontiguousarray(pos4, np.float32).copy()
+    fixf = np.ascontiguousarray(fixf, np.int64)
+    bx = np.ascontiguousarray(box, np.float32)
+    lib().or_minimize_update(pos4.shape[0], _p(pos4, F), _p(fixf, I64), float(gamma),
+                             float(dmax), _p(bx, F))
+    return pos4

@@ -73,6 +73,7 @@ static int eval_pairs_common(const md_rl_table *t, const float *ref, const float
                              const float *pa, const float *pb, const float *pqq, float *fs,
                              float *fxyz, int64_t n, float rc2, int flags)
 {
+    if (n == 0) return MDRL_OK;
     if (!t || !ref || !nbr || !fxyz || n < 0) { set_error("md_rl_eval_pairs: bad argument"); return MDRL_EINVAL; }
     if ((flags & MDRL_PAIR_COULOMB) && !t->has[MDRL_TERM_3]) {
         set_error("md_rl_eval_pairs: Coulomb requested but the table has no r^-3 term");
@@ -130,6 +131,7 @@ extern "C" int md_rl_eval_pairs(const md_rl_table *t, const int32_t *ref_id, con
                                 float rc2, int flags)
 {
     (void)ref_id; (void)nbr_id;   // carried by the reference's kernel, unused by the force
+    if (npairs == 0 && t) return MDRL_OK;
     if (npairs < 0 || !ref_x || !ref_y || !ref_z || !nbr_x || !nbr_y || !nbr_z || !force_x ||
         !force_y || !force_z) {
         set_error("md_rl_eval_pairs: bad argument");
@@ -540,6 +542,25 @@ extern "C" int md_rl_step(md_rl_ctx *c, int nsteps)
     int rc = md_rl_step_async(c, nsteps, nullptr);
     if (rc) return rc;
     return md_rl_sync(c);
+}
+
+extern "C" int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, float gamma)
+{
+    if (!c || !c->have_particles || iterations < 0 || !(max_disp > 0) || !(gamma > 0)) {
+        set_error("md_rl_minimize: bad argument");
+        return MDRL_EINVAL;
+    }
+    DeviceSystem &s = c->s;
+    int rc;
+    for (int k = 0; k < iterations; ++k) {
+        if ((rc = launch_minimize(s, gamma, max_disp, c->stream)) || (rc = launch_rebin(s, c->stream)) ||
+            (rc = launch_zero_forces(s, c->stream, false)) || (rc = launch_force(s, c->stream)))
+            return rc;
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    s.first_step = true;
+    return check_flags(c);
 }
 
 extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)

@@ -83,6 +83,7 @@ int launch_bin_initial(DeviceSystem &s, hipStream_t st);   // cell ids of the cu
 int launch_rebin(DeviceSystem &s, hipStream_t st);         // scan, scatter, sort, gather
 int launch_force(DeviceSystem &s, hipStream_t st);
 int launch_update(DeviceSystem &s, float dt, hipStream_t st);
+int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
 int launch_energy(DeviceSystem &s, hipStream_t st);
 int launch_zero_forces(DeviceSystem &s, hipStream_t st, bool forces);  // pair counters (+ forces)
 int launch_zero_red(DeviceSystem &s, hipStream_t st);     // FP64 reductions

@@ -567,10 +567,17 @@ int launch_rebin(DeviceSystem &s, hipStream_t st)
 // The kinetic energy of the full-step velocity is reduced in FP64.
 // ---------------------------------------------------------------------------------
 
+// Periodic wrap into [0, L): one add or subtract of L (the reference's wrap, C model
+// :175-192), then a floor-based wrap for the rare atom that moved more than a box
+// length; a non-finite coordinate becomes 0 (the caller has flagged it already).
 __device__ __forceinline__ float wrap1(float x, float L)
 {
     if (x < 0.0f) x = x + L;
     if (x >= L) x = x - L;
+    if (!(x >= 0.0f && x < L)) {
+        x = x - L * floorf(x / L);
+        if (!(x >= 0.0f && x < L)) x = 0.0f;
+    }
     return x;
 }
 
@@ -618,9 +625,9 @@ __global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel,
             }
             float step = vn * dt;
             float x = xi[d] + step;
+            if (!isfinite(x) || !isfinite(vn)) flag |= 2u;
             xi[d] = wrap1(x, L[d]);
             vi[d] = vn;
-            if (!isfinite(xi[d]) || !isfinite(vn)) flag |= 2u;
         }
         ke = 0.5 * mass[t] * vv * 1.0e4;
         pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
@@ -652,6 +659,58 @@ int launch_update(DeviceSystem &s, float dt, hipStream_t st)
                        (s.flags & MDRL_KICK_DRIFT) ? 1 : 0, s.cell, s.cell_next, s.red, s.counters);
     MDRL_HIP(hipGetLastError());
     s.first_step = false;
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// k_minimize: capped steepest descent, the untimed pre-relaxation of lattice systems
+// whose jitter leaves overlapping atoms.  Per atom: F in kJ/mol/nm (int64 * 2^-32),
+// s = min(gamma, dmax / |F|) in double, x' = wrap(x + (float)(F * s)).
+// ---------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_minimize(int n, float4 *pos, const long long *fix,
+                                                  const float *f32, double gamma, double dmax,
+                                                  float b0, float b1, float b2, float ie0,
+                                                  float ie1, float ie2, int nc0, int nc1, int nc2,
+                                                  int *cell, int *count,
+                                                  unsigned long long *counters)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double f[3];
+    for (int d = 0; d < 3; ++d)
+        f[d] = f32 ? (double)f32[4 * (size_t)s + d]
+                   : (double)fix[3 * (size_t)s + d] * (1.0 / 4294967296.0);
+    const double fm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    const double sc = (fm * gamma > dmax) ? dmax / fm : gamma;
+    float4 p = pos[s];
+    float xi[3] = {p.x, p.y, p.z};
+    const float L[3] = {b0, b1, b2};
+    unsigned flag = 0;
+    for (int d = 0; d < 3; ++d) {
+        float step = (float)(f[d] * sc);
+        float x = xi[d] + step;
+        if (!isfinite(x)) flag |= 2u;
+        xi[d] = wrap1(x, L[d]);
+    }
+    pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
+    int c = (cell_of(xi[0], ie0, nc0) * nc1 + cell_of(xi[1], ie1, nc1)) * nc2 + cell_of(xi[2], ie2, nc2);
+    cell[s] = c;
+    atomicAdd(&count[c], 1);
+    if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st)
+{
+    const Geometry &g = s.g;
+    if (g.n == 0) return MDRL_OK;
+    const bool f32 = s.flags & MDRL_ACC_FP32;
+    hipLaunchKernelGGL(k_minimize, dim3(nblk(g.n, 256)), dim3(256), 0, st, g.n, s.pos,
+                       f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, gamma, dmax, g.box[0],
+                       g.box[1], g.box[2], g.inv_edge[0], g.inv_edge[1], g.inv_edge[2], g.nc[0],
+                       g.nc[1], g.nc[2], s.cell, s.cell_next, s.counters);
+    MDRL_HIP(hipGetLastError());
     return MDRL_OK;
 }
 

@@ -108,6 +108,7 @@ def lib():
         L.md_rl_step.argtypes = [P, I]
         L.md_rl_step_async.argtypes = [P, I, P]
         L.md_rl_sync.argtypes = [P]
+        L.md_rl_minimize.argtypes = [P, I, ctypes.c_float, ctypes.c_float]
         L.md_rl_get_forces_fixed.argtypes = [P, ctypes.POINTER(ctypes.c_int64), I]
         L.md_rl_energy.argtypes = [P, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -299,6 +300,9 @@ class System:
 
     def sync(self):
         _check(lib().md_rl_sync(self.handle))
+
+    def minimize(self, iterations, max_disp=0.05, gamma=1e-4):
+        _check(lib().md_rl_minimize(self.handle, iterations, max_disp, gamma))
 
     def particles(self):
         out = np.zeros((self.n, 11), np.float32)
