@@ -60,8 +60,10 @@ constexpr double kCoulomb = 1389.35;      // kJ A / (mol e^2)
 // Mixed-species recipe: types with fractions 0.50/0.25/0.15/0.10.
 const float kSig4[4] = {1.0f, 3.2f, 3.5f, 3.0f};
 const float kEps4[4] = {0.19f, 0.65f, 0.30f, 0.71f};
-// Masses are not part of the workload definition; H, C, N, O are used.
-const float kMass4[4] = {1.008f, 12.011f, 14.007f, 15.999f};
+// Masses are not part of the workload definition.  The light type (sigma 1 A) gets a
+// carbon mass: at 1 amu, its Coulomb-bound contacts (r ~ 0.65-1.3 A) vibrate with
+// omega*dt ~ 3 at dt = 1 fs, where velocity Verlet is unstable (omega*dt < 2 needed).
+const float kMass4[4] = {12.011f, 12.011f, 14.007f, 15.999f};
 
 void shuffle(std::vector<int> &v, Rng &r)
 {

@@ -100,13 +100,40 @@ __device__ __forceinline__ float pair_geom(float4 pi, float4 pj, float sx, float
     }
 }
 
+// Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3).  Segments inside the LDS
+// window are read from LDS; the rare r2 outside it (overlapping atoms, or the
+// reference's segment-0 fallback) re-reads the entry from the global copy.
+template <bool COUL, int STR>
+__device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const float *__restrict__ g_tab,
+                                             int off, int nseg, int seg_lo, int seg_hi, float c[6])
+{
+    const int seg = lut_seg(r2, off, nseg);
+    const int idx = seg * 256 + lut_bin(r2);
+    const bool inw = (seg >= seg_lo) && (seg <= seg_hi);
+    const int li = inw ? idx - seg_lo * 256 : 0;
+    if (STR == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(s_tab + li * 4);
+        c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
+    } else {
+        const float2 u = *reinterpret_cast<const float2 *>(s_tab + li * 6);
+        const float2 v = *reinterpret_cast<const float2 *>(s_tab + li * 6 + 2);
+        const float2 w = *reinterpret_cast<const float2 *>(s_tab + li * 6 + 4);
+        c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y; c[4] = w.x; c[5] = w.y;
+    }
+    if (!inw) {
+        const float *q = g_tab + (size_t)idx * 6;
+        c[0] = q[0]; c[1] = q[1]; c[2] = q[2]; c[3] = q[3];
+        if (COUL) { c[4] = q[4]; c[5] = q[5]; }
+    }
+}
+
 // Scalar pair force F (kJ/mol/nm per A of separation): table path
 // D/RL_LJ_Evaluate_Pairs_1st_Order.v:409-475 with the per-pair multipliers of
 // D/RL_Evaluate_Pairs_1st_Order.v:53-55 (A = 10 eps sig^12, B = 10 eps sig^6, qq = qs_i qs_j).
 template <bool COUL, bool PAR, bool ANA, int STR>
 __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float qq,
-                                             const float *s_tab, const float *g_tab, int off,
-                                             int nseg, int seg_lo, int seg_hi)
+                                             const float *s_tab, const float *__restrict__ g_tab,
+                                             int off, int nseg, int seg_lo, int seg_hi)
 {
     if (ANA) {
         float ir2 = 1.0f / r2;
@@ -122,40 +149,25 @@ __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float q
         }
         return F;
     }
-    int seg = lut_seg(r2, off, nseg);
-    int idx = seg * 256 + lut_bin(r2);
-    float c0a, c1a, c0b, c1b, c0c = 0.f, c1c = 0.f;
-    if (seg >= seg_lo && seg <= seg_hi) {
-        const float *p = s_tab + (idx - seg_lo * 256) * STR;
-        if (STR == 4) {
-            float4 v = *(const float4 *)p;
-            c0a = v.x; c1a = v.y; c0b = v.z; c1b = v.w;
-        } else {
-            float2 u = *(const float2 *)p, v = *(const float2 *)(p + 2), w = *(const float2 *)(p + 4);
-            c0a = u.x; c1a = u.y; c0b = v.x; c1b = v.y; c0c = w.x; c1c = w.y;
-        }
-    } else {
-        const float *p = g_tab + (size_t)idx * 6;
-        c0a = p[0]; c1a = p[1]; c0b = p[2]; c1b = p[3];
-        if (COUL) { c0c = p[4]; c1c = p[5]; }
-    }
+    float c[6];
+    table_coeffs<COUL, STR>(r2, s_tab, g_tab, off, nseg, seg_lo, seg_hi, c);
     float F;
     if (PAR) {
-        float t14 = c1a * r2; t14 = t14 + c0a;
-        float t8 = c1b * r2;  t8 = t8 + c0b;
+        float t14 = c[1] * r2; t14 = t14 + c[0];
+        float t8 = c[3] * r2;  t8 = t8 + c[2];
         float x = A * t14, y = B * t8;
         F = x - y;
         if (COUL) {
-            float t3 = c1c * r2; t3 = t3 + c0c;
+            float t3 = c[5] * r2; t3 = t3 + c[4];
             float z = qq * t3;
             F = F + z;
         }
     } else {
-        float t14 = fmaf(c1a, r2, c0a);
-        float t8 = fmaf(c1b, r2, c0b);
+        float t14 = fmaf(c[1], r2, c[0]);
+        float t8 = fmaf(c[3], r2, c[2]);
         F = fmaf(A, t14, -(B * t8));
         if (COUL) {
-            float t3 = fmaf(c1c, r2, c0c);
+            float t3 = fmaf(c[5], r2, c[4]);
             F = fmaf(qq, t3, F);
         }
     }
@@ -166,20 +178,21 @@ __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float q
 // k_rl_halfshell: one workgroup per (home cell, group of half-shell offsets).
 //
 // The home cell's atoms are staged in LDS (tile of TI), each neighbour cell of the group
-// is staged (pre-shifted by its periodic image) as a tile of TJ.  Each wave takes
-// 64-atom chunks of the home tile, one atom per lane, and streams the neighbour tile
-// through LDS broadcasts: the r2 <= rc2 test of D/Filter_Logic.v:234 becomes a ballot,
-// and the passing (i, j) pairs are compacted into a per-wave LDS ring (the filter
-// buffers + arbiter of D/Filter_Bank.v:548-606).  Whenever 64 pairs are queued, all 64
-// lanes evaluate them densely: table force in LDS, then the reference particle gets +F
-// and the neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed
-// point (exact, order-free) or FP32.  Neighbour forces are flushed to HBM with one
-// atomic per touched particle and component after each neighbour tile
-// (D/Force_Write_Back_Controller.v:370-429), home forces after the last offset
+// is staged (pre-shifted by its periodic image) as a tile of TJ.  The four waves split
+// the neighbour tile into quarters and each streams its quarter, one neighbour per step
+// as an LDS broadcast, against every 64-atom chunk of the home tile (one home atom per
+// lane): the r2 <= rc2 test of D/Filter_Logic.v:234 becomes a ballot, and the passing
+// (i, j) pairs are compacted into a per-wave LDS ring (the filter buffers + arbiter of
+// D/Filter_Bank.v:548-606).  Whenever 64 pairs are queued, all 64 lanes evaluate them
+// densely: table force from an LDS window, then the reference particle gets +F and the
+// neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed point
+// (exact, order-free) or FP32.  Neighbour forces are flushed to HBM with one atomic per
+// touched particle and component after each neighbour tile
+// (D/Force_Write_Back_Controller.v:370-429), home forces after the group's last offset
 // (D/Partial_Force_Acc.v:65-129).
 // ---------------------------------------------------------------------------------
 
-constexpr int TI = 256, TJ = 256, QCAP = 128, NWAVE = 4, NTHR = 256;
+constexpr int TI = 256, TJ = 256, QCAP = 512, NWAVE = 4, NTHR = 256, UNROLL = 4;
 
 struct ForceArgs {
     const float4 *pos;
@@ -243,11 +256,9 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
     for (int itile = 0; itile < hcount; itile += TI) {
         const int mi = min(TI, hcount - itile);
         __syncthreads();
-        for (int k = tid; k < TI; k += NTHR) {
-            if (k < mi) {
-                s_i[k] = a.pos[hstart + itile + k];
-                s_it[k] = a.type[hstart + itile + k];
-            }
+        for (int k = tid; k < mi; k += NTHR) {
+            s_i[k] = a.pos[hstart + itile + k];
+            s_it[k] = a.type[hstart + itile + k];
         }
         for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
 
@@ -302,35 +313,64 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
                     }
                 };
 
-                for (int chunk = wave; chunk * 64 < mi; chunk += NWAVE) {
+                // this wave's quarter of the neighbour tile
+                const int jper = (mj + NWAVE - 1) / NWAVE;
+                const int jb0 = min(mj, wave * jper), jb1 = min(mj, jb0 + jper);
+                unsigned head = 0, tail = 0;
+
+                auto test = [&](const float4 &pi, bool vi, int gi, float4 pj, int jb, bool slf) {
+                    float dx, dy, dz;
+                    const float r2 = pair_geom<PAR>(pi, pj, sx, sy, sz, dx, dy, dz);
+                    bool pass = vi && (r2 > 0.0f) && (r2 <= rc2);
+                    if (slf) pass = pass && (gi < jtile + jb);
+                    const unsigned long long m = __ballot(pass);
+                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)(gi - itile) | ((unsigned)jb << 16);
+                    tail += (unsigned)__popcll(m);
+                };
+                auto drain = [&]() {
+                    while (tail - head >= 64u) {
+                        __builtin_amdgcn_wave_barrier();
+                        eval(q[(head + lane) & (QCAP - 1)]);
+                        __builtin_amdgcn_wave_barrier();
+                        head += 64u;
+                    }
+                };
+
+                for (int chunk = 0; chunk * 64 < mi; ++chunk) {
+                    if (self && itile + chunk * 64 >= jtile + jb1) break;   // whole chunk above the diagonal
                     const int ia = chunk * 64 + lane;
                     const bool vi = ia < mi;
                     const float4 pi = s_i[vi ? ia : 0];
                     const int gi = itile + ia;
-                    unsigned head = 0, tail = 0;
-                    for (int jb = 0; jb < mj; ++jb) {
-                        const float4 pj = s_j[jb];
-                        float dx, dy, dz;
-                        const float r2 = pair_geom<PAR>(pi, pj, sx, sy, sz, dx, dy, dz);
-                        bool pass = vi && (r2 > 0.0f) && (r2 <= rc2);
-                        if (self) pass = pass && (gi < jtile + jb);
-                        const unsigned long long m = __ballot(pass);
-                        if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
-                        tail += (unsigned)__popcll(m);
-                        if (tail - head >= 64u) {
-                            __builtin_amdgcn_wave_barrier();
-                            eval(q[(head + lane) & (QCAP - 1)]);
-                            __builtin_amdgcn_wave_barrier();
-                            head += 64u;
+                    int jb = jb0;
+                    if (self) {
+                        for (; jb + UNROLL <= jb1; jb += UNROLL) {
+                            float4 pj[UNROLL];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; ++u) pj[u] = s_j[jb + u];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; ++u) test(pi, vi, gi, pj[u], jb + u, true);
+                            drain();
                         }
+                        for (; jb < jb1; ++jb) { test(pi, vi, gi, s_j[jb], jb, true); drain(); }
+                    } else {
+                        for (; jb + UNROLL <= jb1; jb += UNROLL) {
+                            float4 pj[UNROLL];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; ++u) pj[u] = s_j[jb + u];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; ++u) test(pi, vi, gi, pj[u], jb + u, false);
+                            drain();
+                        }
+                        for (; jb < jb1; ++jb) { test(pi, vi, gi, s_j[jb], jb, false); drain(); }
                     }
-                    ncand += (unsigned long long)mj;
-                    npairs += tail;
-                    if (tail != head) {
-                        __builtin_amdgcn_wave_barrier();
-                        if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
-                        __builtin_amdgcn_wave_barrier();
-                    }
+                    ncand += (unsigned long long)(jb1 - jb0);
+                }
+                npairs += tail;
+                if (tail != head) {
+                    __builtin_amdgcn_wave_barrier();
+                    if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
+                    __builtin_amdgcn_wave_barrier();
                 }
                 __syncthreads();
                 // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)

@@ -232,8 +232,9 @@ def test_minimize_bit_exact(apoa1, which):
     g = gpu_system(s)
     rl = rl_from_synthetic(s, apoa1)
     fix, _ = rl.forces_fix()
+    gamma, dmax = float(np.float32(1e-4)), float(np.float32(0.05))   # the ABI takes floats
     for _ in range(4):
-        rl.pos4[:] = O.minimize_update(rl.pos4, fix, 1e-4, 0.05, s.box)
+        rl.pos4[:] = O.minimize_update(rl.pos4, fix, gamma, dmax, s.box)
         fix, _ = rl.forces_fix()
     g.minimize(4, 0.05, 1e-4)
     got = g.particles()
