@@ -264,9 +264,17 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
         delete c;
         return MDRL_EINVAL;
     }
-    s.win.seg_hi = shi < 0 ? 0 : shi;
-    s.win.seg_lo = s.win.seg_hi >= 8 ? s.win.seg_hi - 8 : 0;
-    s.win.stride = s.coul ? 6 : 4;
+    {
+        // LDS window: from seven octaves below rc^2 up to rc^2's own bin
+        uint32_t u;
+        memcpy(&u, &g.rc2, 4);
+        const int hi_seg = shi < 0 ? 0 : shi;
+        const int lo_seg = hi_seg >= 7 ? hi_seg - 7 : 0;
+        const int hi_idx = hi_seg * 256 + (int)((u >> 15) & 0xff);
+        s.win.lo = lo_seg * 256;
+        s.win.n = hi_idx - s.win.lo + 1;
+        s.win.stride = s.coul ? 6 : 4;
+    }
     c->qscale = s.coul ? sqrt(10.0 * (double)p->coulomb_k * 4.0 * 3.14) : 0.0;
 
     const int n = g.n, nt = p->ntypes;

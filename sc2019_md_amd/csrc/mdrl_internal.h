@@ -40,9 +40,9 @@ struct Geometry {
     float rc2 = 0;
 };
 
-// Table window kept in LDS by the force kernel.
+// Table window kept in LDS by the force kernel: entries [lo, lo + n).
 struct TableWindow {
-    int seg_lo = 0, seg_hi = 0;         // inclusive segment range held in LDS
+    int lo = 0, n = 0;
     int stride = 6;                     // floats per entry (6 with r^-3, 4 without)
 };
 

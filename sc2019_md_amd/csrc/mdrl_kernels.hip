@@ -48,6 +48,30 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// Counting-sort increments with one atomic per run of equal keys in consecutive lanes:
+// slots stay cell-sorted between steps, so a wave's atoms mostly share one or two cells.
+// Active lanes must form a prefix of the wave.  Returns, for each active lane, the old
+// value of count[key] plus the lane's rank within its run (its slot for a scatter).
+__device__ __forceinline__ int run_add(int *count, int key, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long act = __ballot(active);
+    const int nact = __popcll(act);
+    const bool head = active && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int base = 0;
+    if (head) {
+        const unsigned long long after = heads & ~upto;
+        const int end = after ? __ffsll((long long)after) - 1 : nact;
+        base = atomicAdd(&count[key], end - lane);
+    }
+    const int hl = 63 - __clzll(heads & upto);
+    const int hb = __shfl(base, active ? hl : lane);
+    return hb + (lane - hl);
+}
+
 template <typename T>
 __device__ __forceinline__ void atomic_add_agent(T *p, T v)
 {
@@ -57,6 +81,11 @@ template <typename T>
 __device__ __forceinline__ void atomic_add_wg(T *p, T v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void atomic_sub_wg(T *p, T v)
+{
+    __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Cell of one coordinate: (int)(x * inv_edge) clamped to [0, nc-1]; the clamp comes
@@ -100,17 +129,18 @@ __device__ __forceinline__ float pair_geom(float4 pi, float4 pj, float sx, float
     }
 }
 
-// Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3).  Segments inside the LDS
-// window are read from LDS; the rare r2 outside it (overlapping atoms, or the
-// reference's segment-0 fallback) re-reads the entry from the global copy.
+// Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3).  Entries [win_lo, win_lo + nwin)
+// -- the bins reachable below rc2 down to a floor seven octaves lower -- are read from
+// LDS; the rare r2 outside (overlapping atoms, or the reference's segment-0 fallback)
+// re-reads its entry from the global copy.
 template <bool COUL, int STR>
 __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const float *__restrict__ g_tab,
-                                             int off, int nseg, int seg_lo, int seg_hi, float c[6])
+                                             int off, int nseg, int win_lo, int nwin, float c[6])
 {
     const int seg = lut_seg(r2, off, nseg);
     const int idx = seg * 256 + lut_bin(r2);
-    const bool inw = (seg >= seg_lo) && (seg <= seg_hi);
-    const int li = inw ? idx - seg_lo * 256 : 0;
+    const bool inw = (unsigned)(idx - win_lo) < (unsigned)nwin;
+    const int li = inw ? idx - win_lo : 0;
     if (STR == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(s_tab + li * 4);
         c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
@@ -133,7 +163,7 @@ __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const
 template <bool COUL, bool PAR, bool ANA, int STR>
 __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float qq,
                                              const float *s_tab, const float *__restrict__ g_tab,
-                                             int off, int nseg, int seg_lo, int seg_hi)
+                                             int off, int nseg, int win_lo, int nwin)
 {
     if (ANA) {
         float ir2 = 1.0f / r2;
@@ -150,7 +180,7 @@ __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float q
         return F;
     }
     float c[6];
-    table_coeffs<COUL, STR>(r2, s_tab, g_tab, off, nseg, seg_lo, seg_hi, c);
+    table_coeffs<COUL, STR>(r2, s_tab, g_tab, off, nseg, win_lo, nwin, c);
     float F;
     if (PAR) {
         float t14 = c[1] * r2; t14 = t14 + c[0];
@@ -175,24 +205,24 @@ __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float q
 }
 
 // ---------------------------------------------------------------------------------
-// k_rl_halfshell: one workgroup per (home cell, group of half-shell offsets).
+// k_rl_halfshell: one workgroup of 8 waves per (home cell, group of half-shell offsets).
 //
-// The home cell's atoms are staged in LDS (tile of TI), each neighbour cell of the group
-// is staged (pre-shifted by its periodic image) as a tile of TJ.  The four waves split
-// the neighbour tile into quarters and each streams its quarter, one neighbour per step
-// as an LDS broadcast, against every 64-atom chunk of the home tile (one home atom per
-// lane): the r2 <= rc2 test of D/Filter_Logic.v:234 becomes a ballot, and the passing
-// (i, j) pairs are compacted into a per-wave LDS ring (the filter buffers + arbiter of
-// D/Filter_Bank.v:548-606).  Whenever 64 pairs are queued, all 64 lanes evaluate them
-// densely: table force from an LDS window, then the reference particle gets +F and the
-// neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed point
-// (exact, order-free) or FP32.  Neighbour forces are flushed to HBM with one atomic per
-// touched particle and component after each neighbour tile
-// (D/Force_Write_Back_Controller.v:370-429), home forces after the group's last offset
-// (D/Partial_Force_Acc.v:65-129).
+// The home cell's atoms are staged in LDS (tile of TI = 256), each neighbour cell of the
+// group is staged (pre-shifted by its periodic image) as a tile of TJ.  Wave w streams
+// neighbours w, w+8, w+16, ... of the tile, one per step as an LDS broadcast, against
+// every 64-atom chunk of the home tile (one home atom per lane, up to four chunks held in
+// registers, so one neighbour read feeds four independent tests).  The r2 <= rc2 test of
+// D/Filter_Logic.v:234 is a ballot; passing (i, j) pairs are compacted into a per-wave LDS
+// ring (the filter buffers + arbiter of D/Filter_Bank.v:548-606) and, whenever 64 are
+// queued, all 64 lanes evaluate them densely: table force from an LDS window, then the
+// reference particle gets +F and the neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS
+// accumulators, int64 fixed point (exact, order-free) or FP32.  Neighbour forces are
+// flushed to HBM with one atomic per touched particle and component after each
+// neighbour tile (D/Force_Write_Back_Controller.v:370-429), home forces after the group's
+// last offset (D/Partial_Force_Acc.v:65-129).
 // ---------------------------------------------------------------------------------
 
-constexpr int TI = 256, TJ = 256, QCAP = 512, NWAVE = 4, NTHR = 256, UNROLL = 4;
+constexpr int TI = 256, TJ = 256, QCAP = 128, NWAVE = 8, NTHR = 512, NCHUNK = TI / 64;
 
 struct ForceArgs {
     const float4 *pos;
@@ -202,7 +232,7 @@ struct ForceArgs {
     float box0, box1, box2;
     float rc2;
     const float *tab;
-    int off, nseg, seg_lo, seg_hi;
+    int off, nseg, win_lo, nwin;
     const float2 *pair_ab;
     int ntypes;
     long long *fix;
@@ -226,7 +256,8 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
     __shared__ unsigned s_q[NWAVE][QCAP];
     __shared__ float2 s_ab[64];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
     const int task = blockIdx.x;
     const int h = task / a.ngroups, grp = task - h * a.ngroups;
     const int hz = h % a.nc2, hy = (h / a.nc2) % a.nc1, hx = h / (a.nc1 * a.nc2);
@@ -235,9 +266,8 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
 
     // table window and type-pair multipliers
     if (!ANA) {
-        const int nw = (a.seg_hi - a.seg_lo + 1) * 256;
-        const float *g = a.tab + (size_t)a.seg_lo * 256 * 6;
-        for (int e = tid; e < nw; e += NTHR) {
+        const float *g = a.tab + (size_t)a.win_lo * 6;
+        for (int e = tid; e < a.nwin; e += NTHR) {
             const float *p = g + (size_t)e * 6;
             float *q = s_tab + e * STR;
             q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[3];
@@ -255,12 +285,22 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
 
     for (int itile = 0; itile < hcount; itile += TI) {
         const int mi = min(TI, hcount - itile);
+        const int nch = (mi + 63) >> 6;
         __syncthreads();
         for (int k = tid; k < mi; k += NTHR) {
             s_i[k] = a.pos[hstart + itile + k];
             s_it[k] = a.type[hstart + itile + k];
         }
         for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
+        __syncthreads();
+        // this lane's home atoms, one per chunk; lanes past the tile hold a far-away
+        // point (r2 = inf never passes)
+        float4 pi[NCHUNK];
+#pragma unroll
+        for (int ch = 0; ch < NCHUNK; ++ch) {
+            const int ia = ch * 64 + lane;
+            pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+        }
 
         for (int o = a.gstart[grp]; o < a.gstart[grp + 1]; ++o) {
             const bool self = (o == 0);
@@ -287,13 +327,13 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
 
                 auto eval = [&](unsigned e) {
                     const int ia = (int)(e & 0xffffu), jb = (int)(e >> 16);
-                    const float4 pi = s_i[ia], pj = s_j[jb];
+                    const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
-                    const float r2 = pair_geom<PAR>(pi, pj, sx, sy, sz, dx, dy, dz);
+                    const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
                     const float2 ab = s_ab[s_it[ia] * a.ntypes + s_jt[jb]];
-                    const float qq = pi.w * pj.w;
+                    const float qq = pa.w * pb.w;
                     const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
-                                                                     a.off, a.nseg, a.seg_lo, a.seg_hi);
+                                                                     a.off, a.nseg, a.win_lo, a.nwin);
                     const float fx = F * dx, fy = F * dy, fz = F * dz;
                     if (F32) {
                         atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
@@ -307,65 +347,47 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
                         atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
                         atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
                         atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
-                        atomic_add_wg(&s_jacc[3 * jb + 0], (unsigned long long)(-kx));
-                        atomic_add_wg(&s_jacc[3 * jb + 1], (unsigned long long)(-ky));
-                        atomic_add_wg(&s_jacc[3 * jb + 2], (unsigned long long)(-kz));
+                        atomic_sub_wg(&s_jacc[3 * jb + 0], (unsigned long long)kx);
+                        atomic_sub_wg(&s_jacc[3 * jb + 1], (unsigned long long)ky);
+                        atomic_sub_wg(&s_jacc[3 * jb + 2], (unsigned long long)kz);
                     }
                 };
 
-                // this wave's quarter of the neighbour tile
-                const int jper = (mj + NWAVE - 1) / NWAVE;
-                const int jb0 = min(mj, wave * jper), jb1 = min(mj, jb0 + jper);
                 unsigned head = 0, tail = 0;
-
-                auto test = [&](const float4 &pi, bool vi, int gi, float4 pj, int jb, bool slf) {
-                    float dx, dy, dz;
-                    const float r2 = pair_geom<PAR>(pi, pj, sx, sy, sz, dx, dy, dz);
-                    bool pass = vi && (r2 > 0.0f) && (r2 <= rc2);
-                    if (slf) pass = pass && (gi < jtile + jb);
-                    const unsigned long long m = __ballot(pass);
-                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)(gi - itile) | ((unsigned)jb << 16);
-                    tail += (unsigned)__popcll(m);
-                };
-                auto drain = [&]() {
-                    while (tail - head >= 64u) {
-                        __builtin_amdgcn_wave_barrier();
-                        eval(q[(head + lane) & (QCAP - 1)]);
-                        __builtin_amdgcn_wave_barrier();
-                        head += 64u;
-                    }
-                };
-
-                for (int chunk = 0; chunk * 64 < mi; ++chunk) {
-                    if (self && itile + chunk * 64 >= jtile + jb1) break;   // whole chunk above the diagonal
-                    const int ia = chunk * 64 + lane;
-                    const bool vi = ia < mi;
-                    const float4 pi = s_i[vi ? ia : 0];
-                    const int gi = itile + ia;
-                    int jb = jb0;
-                    if (self) {
-                        for (; jb + UNROLL <= jb1; jb += UNROLL) {
-                            float4 pj[UNROLL];
+                // Rotation: at step sft, lane l tests neighbour (l + sft) mod mj against each
+                // of its home atoms, so within a step both the home and the neighbour indices
+                // of the queued pairs are distinct (conflict-free accumulator updates) and one
+                // LDS read of the neighbour feeds every chunk.  Steps are strided over waves.
+                const int base = lane % mj;
+                // home-cell pairs are taken once, as (lower, higher) in-cell index: a pair
+                // passes only if its neighbour index exceeds lim (lim = -1 off the diagonal)
+                int lim[NCHUNK];
 #pragma unroll
-                            for (int u = 0; u < UNROLL; ++u) pj[u] = s_j[jb + u];
+                for (int ch = 0; ch < NCHUNK; ++ch) lim[ch] = self ? itile + ch * 64 + lane - jtile : -1;
+                for (int sft = wave; sft < mj; sft += NWAVE) {
+                    int jb = base + sft;
+                    jb = (jb >= mj) ? jb - mj : jb;
+                    const float4 pj = s_j[jb];
 #pragma unroll
-                            for (int u = 0; u < UNROLL; ++u) test(pi, vi, gi, pj[u], jb + u, true);
-                            drain();
+                    for (int ch = 0; ch < NCHUNK; ++ch) {
+                        if (ch < nch) {
+                            float dx, dy, dz;
+                            const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
+                            const int ia = ch * 64 + lane;
+                            const bool pass = (r2 > 0.0f) && (r2 <= rc2) && (jb > lim[ch]);
+                            const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                            if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
+                            tail += (unsigned)__popcll(m);
+                            if (tail - head >= 64u) {
+                                __builtin_amdgcn_wave_barrier();
+                                eval(q[(head + lane) & (QCAP - 1)]);
+                                __builtin_amdgcn_wave_barrier();
+                                head += 64u;
+                            }
                         }
-                        for (; jb < jb1; ++jb) { test(pi, vi, gi, s_j[jb], jb, true); drain(); }
-                    } else {
-                        for (; jb + UNROLL <= jb1; jb += UNROLL) {
-                            float4 pj[UNROLL];
-#pragma unroll
-                            for (int u = 0; u < UNROLL; ++u) pj[u] = s_j[jb + u];
-#pragma unroll
-                            for (int u = 0; u < UNROLL; ++u) test(pi, vi, gi, pj[u], jb + u, false);
-                            drain();
-                        }
-                        for (; jb < jb1; ++jb) { test(pi, vi, gi, s_j[jb], jb, false); drain(); }
                     }
-                    ncand += (unsigned long long)(jb1 - jb0);
                 }
+                ncand += (unsigned long long)((mj - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
                 npairs += tail;
                 if (tail != head) {
                     __builtin_amdgcn_wave_barrier();
@@ -430,7 +452,7 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.box0 = s.g.box[0]; a.box1 = s.g.box[1]; a.box2 = s.g.box[2];
     a.rc2 = s.g.rc2;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
-    a.seg_lo = s.win.seg_lo; a.seg_hi = s.win.seg_hi;
+    a.win_lo = s.win.lo; a.nwin = s.win.n;
     a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
     a.fix = s.fix; a.f32 = s.f32; a.counters = s.counters;
     a.ngroups = 4;
@@ -438,7 +460,7 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     const bool f32 = s.flags & MDRL_ACC_FP32, par = s.flags & MDRL_EVAL_PARITY,
                ana = s.flags & MDRL_EVAL_ANALYTIC, coul = s.coul;
     const int str = coul ? 6 : 4;
-    size_t lds = ana ? 0 : (size_t)(s.win.seg_hi - s.win.seg_lo + 1) * 256 * str * sizeof(float);
+    size_t lds = ana ? 0 : (size_t)s.win.n * str * sizeof(float);
     dim3 grid(s.g.ncells * a.ngroups), block(NTHR);
 #define MDRL_LAUNCH_RL(C, F, P, A)                                                            \
     hipLaunchKernelGGL((k_rl_halfshell<C, F, P, A>), grid, block, lds, st, a)
@@ -465,12 +487,14 @@ int launch_force(DeviceSystem &s, hipStream_t st)
 __global__ void k_assign(int n, const float4 *pos, float ie0, float ie1, float ie2, int nc0,
                          int nc1, int nc2, int *cell, int *count)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float4 p = pos[s];
-    int c = (cell_of(p.x, ie0, nc0) * nc1 + cell_of(p.y, ie1, nc1)) * nc2 + cell_of(p.z, ie2, nc2);
-    cell[s] = c;
-    atomicAdd(&count[c], 1);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (s < n) {
+        const float4 p = pos[s];
+        c = (cell_of(p.x, ie0, nc0) * nc1 + cell_of(p.y, ie1, nc1)) * nc2 + cell_of(p.z, ie2, nc2);
+        cell[s] = c;
+    }
+    run_add(count, c, s < n);
 }
 
 // Exclusive scan of the next-step counts into cell_start; clears counts and fill
@@ -507,15 +531,17 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
     if (t == 1023) {
         start[ncells] = s_sum[1023];
         counters[3] = (unsigned long long)s_max[1023];
+        counters[0] = 0ull;   // pair and candidate counts of the force evaluation that follows
+        counters[1] = 0ull;
     }
 }
 
 __global__ void k_scatter(int n, const int *cell, const int *start, int *fill, int *perm)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int c = cell[s];
-    perm[start[c] + atomicAdd(&fill[c], 1)] = s;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (s < n) ? cell[s] : 0;
+    const int k = run_add(fill, c, s < n);
+    if (s < n) perm[start[c] + k] = s;
 }
 
 // Orders each cell's slots by input index.  Cells larger than the LDS capacity keep
@@ -633,6 +659,7 @@ __global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel,
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0;
     unsigned flag = 0;
+    int c = 0;
     if (s < n) {
         const int t = type[s];
         const double k = hk[t];
@@ -672,10 +699,10 @@ __global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel,
         ke = 0.5 * mass[t] * vv * 1.0e4;
         pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
         vel[s] = make_float4(vi[0], vi[1], vi[2], (float)ke);
-        int c = (cell_of(xi[0], ie0, nc0) * nc1 + cell_of(xi[1], ie1, nc1)) * nc2 + cell_of(xi[2], ie2, nc2);
+        c = (cell_of(xi[0], ie0, nc0) * nc1 + cell_of(xi[1], ie1, nc1)) * nc2 + cell_of(xi[2], ie2, nc2);
         cell[s] = c;
-        atomicAdd(&count[c], 1);
     }
+    run_add(count, c, s < n);
     s_red[threadIdx.x] = ke;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -716,27 +743,29 @@ __global__ __launch_bounds__(256) void k_minimize(int n, float4 *pos, const long
                                                   unsigned long long *counters)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    double f[3];
-    for (int d = 0; d < 3; ++d)
-        f[d] = f32 ? (double)f32[4 * (size_t)s + d]
-                   : (double)fix[3 * (size_t)s + d] * (1.0 / 4294967296.0);
-    const double fm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-    const double sc = (fm * gamma > dmax) ? dmax / fm : gamma;
-    float4 p = pos[s];
-    float xi[3] = {p.x, p.y, p.z};
-    const float L[3] = {b0, b1, b2};
     unsigned flag = 0;
-    for (int d = 0; d < 3; ++d) {
-        float step = (float)(f[d] * sc);
-        float x = xi[d] + step;
-        if (!isfinite(x)) flag |= 2u;
-        xi[d] = wrap1(x, L[d]);
+    int c = 0;
+    if (s < n) {
+        double f[3];
+        for (int d = 0; d < 3; ++d)
+            f[d] = f32 ? (double)f32[4 * (size_t)s + d]
+                       : (double)fix[3 * (size_t)s + d] * (1.0 / 4294967296.0);
+        const double fm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double sc = (fm * gamma > dmax) ? dmax / fm : gamma;
+        float4 p = pos[s];
+        float xi[3] = {p.x, p.y, p.z};
+        const float L[3] = {b0, b1, b2};
+        for (int d = 0; d < 3; ++d) {
+            float step = (float)(f[d] * sc);
+            float x = xi[d] + step;
+            if (!isfinite(x)) flag |= 2u;
+            xi[d] = wrap1(x, L[d]);
+        }
+        pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
+        c = (cell_of(xi[0], ie0, nc0) * nc1 + cell_of(xi[1], ie1, nc1)) * nc2 + cell_of(xi[2], ie2, nc2);
+        cell[s] = c;
     }
-    pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
-    int c = (cell_of(xi[0], ie0, nc0) * nc1 + cell_of(xi[1], ie1, nc1)) * nc2 + cell_of(xi[2], ie2, nc2);
-    cell[s] = c;
-    atomicAdd(&count[c], 1);
+    run_add(count, c, s < n);
     if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -828,7 +857,7 @@ int launch_energy(DeviceSystem &s, hipStream_t st)
     a.box0 = s.g.box[0]; a.box1 = s.g.box[1]; a.box2 = s.g.box[2];
     a.rc2 = s.g.rc2;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
-    a.seg_lo = s.win.seg_lo; a.seg_hi = s.win.seg_hi;
+    a.win_lo = s.win.lo; a.nwin = s.win.n;
     a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
     a.fix = s.fix; a.f32 = s.f32; a.counters = s.counters; a.ngroups = 1;
     hipLaunchKernelGGL(k_energy, dim3(s.g.ncells), dim3(256), 0, st, a, s.etab, s.pair_ab_e, s.red);
@@ -897,7 +926,8 @@ __global__ __launch_bounds__(256) void k_kinetic(int n, const float4 *vel, const
 
 int launch_zero_forces(DeviceSystem &s, hipStream_t st, bool forces)
 {
-    if (forces && s.g.n > 0) {
+    if (!forces) return MDRL_OK;   // after a rebin: forces and pair counters are already zero
+    if (s.g.n > 0) {
         const bool f32 = s.flags & MDRL_ACC_FP32;
         hipLaunchKernelGGL(k_zero_forces, dim3(nblk(s.g.n, 256)), dim3(256), 0, st, s.g.n,
                            f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr);
