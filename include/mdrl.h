@@ -173,12 +173,56 @@ int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, float gamma);
 /* Raw int64 fixed-point forces (2^-32 kJ/mol/nm units), n x 3, input order. */
 int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n);
 /* FP64 energy of the current positions from the c*_{12,6,1} tables (potential),
- * kinetic energy of the last full-step velocities and the pair virial. */
+ * kinetic energy of the full-step velocities and the pair virial (sum r_ij F_ij), each
+ * reduced in a fixed order so repeated calls give identical bits. */
 int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9]);
 int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *s);
 /* Enables HIP-event timing of each force kernel and full step (off by default). */
 int md_rl_set_timing(md_rl_ctx *c, int on);
 int md_rl_destroy(md_rl_ctx *c);
+
+/* ---- batches of independent systems -------------------------------------------------- */
+/* Many periodic systems of uneven sizes stepped together: every kernel of a step is one
+ * launch over all systems (the force kernel optionally as a persistent grid draining a
+ * largest-first task queue), and the step can be captured in a hipGraph.  Systems share
+ * rc, dt, flags, species and table (common); each has its own atom count and box.  The
+ * md_rl_* calls above apply to the whole batch (md_rl_energy sums over systems); the
+ * single-system calls md_rl_set_particles/set_types/get_particles/get_forces_fixed
+ * require a batch of one. */
+int md_rl_create_batch(md_rl_ctx **out, const md_rl_params *common, int nsys,
+                       const int32_t *n, const float *box /* nsys x 3 */);
+int md_rl_num_systems(const md_rl_ctx *c);
+int md_rl_system_size(const md_rl_ctx *c, int sys, int32_t *n, float box[3], int ncell[3]);
+/* Stages one system's particles (n x 11: positions, velocities) and/or types and charges
+ * (either may be NULL); the next step, force evaluation or read-back uploads and bins
+ * every staged system.  md_rl_commit uploads at once. */
+int md_rl_set_system(md_rl_ctx *c, int sys, const float *particle, const int32_t *type,
+                     const float *charge);
+int md_rl_commit(md_rl_ctx *c);
+int md_rl_get_system(md_rl_ctx *c, int sys, float *particle);
+int md_rl_get_system_forces_fixed(md_rl_ctx *c, int sys, int64_t *fix);
+int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke, double virial[9]);
+
+/* Capture md_rl_step's steps in hipGraphs (two steps per graph launch); off by default. */
+int md_rl_set_graph(md_rl_ctx *c, int on);
+
+/* Force-kernel schedule.  atoms_per_wave: home atoms per 64-lane wavefront (16, 32 or
+ * 64; below 64, 64/apw lanes share a home atom and split its neighbours); tile_depth:
+ * neighbour atoms staged in LDS per pass (128 or 256); offset_groups: workgroups per home
+ * cell, splitting its 14 half-shell offsets (1, 2 or 4); persistent: a fixed grid draining
+ * a largest-first task queue instead of one workgroup per task.  All schedules give
+ * bit-identical int64 forces. */
+typedef struct md_rl_schedule {
+    int atoms_per_wave;
+    int tile_depth;
+    int offset_groups;
+    int persistent;
+} md_rl_schedule;
+int md_rl_set_schedule(md_rl_ctx *c, const md_rl_schedule *s);
+int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *s);
+/* Times every available schedule on the current positions (reps force evaluations
+ * each, HIP events) and keeps the fastest. */
+int md_rl_autotune(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms);
 
 /* ---- the C model entry point ------------------------------------------------------- */
 /* Drop-in for int MD_Evaluation_Model() (R/C_Model_ForceEval/MD_Evaluation_Model.h:1):

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmdrl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mdrl_kernels.hip", "mdrl_api.cpp", "mdrl_io.cpp", "mdrl_gen.cpp"]
+SOURCES = ["mdrl_system.hip", "mdrl_kernels.hip", "mdrl_api.cpp", "mdrl_io.cpp", "mdrl_gen.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950",
@@ -29,19 +29,26 @@ COMMON = [
 ]
 
 
+def _compile(src, objdir, verbose):
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [l for l in (r.stdout + r.stderr).splitlines()
+            if "packed-fp32-ops" not in l and l.strip()]
+    return src, obj, r.returncode, cmd, msgs
+
+
 def build(verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for src in SOURCES:
-        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        results = list(ex.map(lambda s: _compile(s, objdir, verbose), SOURCES))
+    for src, obj, rc, cmd, msgs in results:
         if verbose:
             print(" ".join(cmd))
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        msgs = [l for l in (r.stdout + r.stderr).splitlines()
-                if "packed-fp32-ops" not in l and l.strip()]
-        if r.returncode != 0:
+        if rc != 0:
             sys.stderr.write("\n".join(msgs) + "\n")
             raise RuntimeError("hipcc failed on %s" % src)
         if verbose and msgs:

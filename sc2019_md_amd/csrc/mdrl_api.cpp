@@ -4,6 +4,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -152,49 +154,59 @@ extern "C" int md_rl_eval_pairs(const md_rl_table *t, const int32_t *ref_id, con
 }
 
 // ---------------------------------------------------------------------------------
-// systems
+// systems (a single system is a batch of one)
 // ---------------------------------------------------------------------------------
 
 struct md_rl_ctx {
     md_rl_params p{};
     DeviceSystem s;
     std::vector<float> eps, sigma, mass;
-    std::vector<int32_t> h_type;   // input order
-    std::vector<float> h_q;        // input order, plain charges
-    double qscale = 0.0;           // qs = q * qscale
+    // staged input, concatenated over systems in input order
+    std::vector<float> h_x, h_v;      // n x 3
+    std::vector<int32_t> h_type;      // n
+    std::vector<float> h_q;           // n, plain charges
+    double qscale = 0.0;              // qs = q * qscale
+    bool have_particles = false;
+    bool dirty = false;               // staged input not yet on the device
     hipStream_t stream = nullptr;
     bool timing = false;
-    hipEvent_t ev[4] = {};
     struct StepEvents { hipEvent_t f0, f1, s0, s1; };
-    std::vector<StepEvents> tev;   // per timed step, reused batch to batch
+    std::vector<StepEvents> tev;      // per timed step, reused batch to batch
     int tev_used = 0;
     double force_ms = 0.0, step_ms = 0.0;   // averages over the last synchronised batch
     int timed_steps = 0;
-    bool have_particles = false;
-    float4 *d_tmp4 = nullptr;      // n: scratch for read-back of full-step velocities
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps, one per buffer parity
+    float4 *d_tmp4 = nullptr;         // n: full-step velocities for read-back
     md_rl_table *own_table = nullptr;
 };
+
+static void drop_graphs(md_rl_ctx *c)
+{
+    for (auto &g : c->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
 
 static void free_ctx_device(md_rl_ctx *c)
 {
     DeviceSystem &s = c->s;
-    void *ptrs[] = {s.pos, s.pos2, s.vel, s.vel2, s.type, s.type2, s.orig, s.orig2, s.cell,
-                    s.perm, s.cell_start, s.cell_next, s.cell_fill, s.fix, s.f32, s.tab, s.etab,
-                    s.pair_ab, s.pair_ab_e, s.hk, s.mass, s.counters, s.red, c->d_tmp4};
+    void *ptrs[] = {s.geom, s.cell_sys, s.slot_sys, s.tasks, s.pos, s.pos2, s.vel, s.vel2, s.type,
+                    s.type2, s.orig, s.orig2, s.cell, s.perm, s.cell_start, s.cell_next,
+                    s.cell_fill, s.fix, s.f32, s.tab, s.etab, s.pair_ab, s.pair_ab_e, s.hk,
+                    s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto &t : c->tev)
-        for (hipEvent_t e : {t.f0, t.f1, t.s0, t.s1})
-            if (e) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    drop_graphs(c);
 }
 
 extern "C" int md_rl_destroy(md_rl_ctx *c)
 {
     if (!c) return MDRL_OK;
     free_ctx_device(c);
+    for (auto &t : c->tev)
+        for (hipEvent_t e : {t.f0, t.f1, t.s0, t.s1})
+            if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     md_rl_table_free(c->own_table);
     delete c;
     return MDRL_OK;
@@ -209,15 +221,49 @@ static int table_segment(float r2, int off, int nseg)
     return (seg < 0 || seg >= nseg) ? -1 : seg;
 }
 
-extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
+// Force tasks (home cell x offset group), largest estimated cost first: the persistent
+// kernel drains them from a counter, so the long tasks of dense small boxes start early.
+static int build_tasks(md_rl_ctx *c)
+{
+    DeviceSystem &s = c->s;
+    const int g = s.sched.groups;
+    static const double w1[1] = {13.5}, w2[2] = {6.5, 7.0}, w4[4] = {3.5, 4.0, 3.0, 3.0};
+    const double *w = g == 1 ? w1 : g == 2 ? w2 : w4;
+    std::vector<std::pair<double, int>> cost;
+    cost.reserve((size_t)s.ncells * g);
+    for (const SysGeom &G : s.h_geom) {
+        const int nc = G.nc[0] * G.nc[1] * G.nc[2];
+        const double a = (double)G.n / nc;
+        for (int k = 0; k < nc; ++k)
+            for (int q = 0; q < g; ++q) cost.push_back({-a * a * w[q], (G.cell_off + k) * g + q});
+    }
+    std::stable_sort(cost.begin(), cost.end(),
+                     [](const std::pair<double, int> &x, const std::pair<double, int> &y) { return x.first < y.first; });
+    std::vector<int> t(cost.size());
+    for (size_t k = 0; k < cost.size(); ++k) t[k] = cost[k].second;
+    if (s.tasks) { (void)hipFree(s.tasks); s.tasks = nullptr; }
+    int rc = dalloc(&s.tasks, t.size());
+    if (rc) return rc;
+    MDRL_HIP(hipMemcpy(s.tasks, t.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice));
+    s.ntasks = (int)t.size();
+    return MDRL_OK;
+}
+
+static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const int32_t *nat,
+                         const float *boxes)
 {
     if (!out || !p) { set_error("md_rl_create: NULL argument"); return MDRL_EINVAL; }
     *out = nullptr;
-    if (p->n < 0 || p->rc <= 0 || p->ntypes < 1 || p->ntypes > 8 || !p->eps || !p->sigma || !p->mass ||
-        !(p->box[0] > 0 && p->box[1] > 0 && p->box[2] > 0) || !(p->dt > 0)) {
-        set_error("md_rl_create: invalid parameters (n=%d rc=%g ntypes=%d)", p->n, p->rc, p->ntypes);
+    if (nsys < 1 || !nat || !boxes || p->rc <= 0 || p->ntypes < 1 || p->ntypes > 8 || !p->eps ||
+        !p->sigma || !p->mass || !(p->dt > 0)) {
+        set_error("md_rl_create: invalid parameters (nsys=%d rc=%g ntypes=%d)", nsys, p->rc, p->ntypes);
         return MDRL_EINVAL;
     }
+    for (int r = 0; r < nsys; ++r)
+        if (nat[r] < 0 || !(boxes[3 * r] > 0 && boxes[3 * r + 1] > 0 && boxes[3 * r + 2] > 0)) {
+            set_error("md_rl_create: invalid system %d (n=%d)", r, nat[r]);
+            return MDRL_EINVAL;
+        }
     int rc = check_device();
     if (rc) return rc;
     md_rl_ctx *c = new md_rl_ctx;
@@ -227,22 +273,34 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     c->mass.assign(p->mass, p->mass + p->ntypes);
     c->p.eps = c->eps.data(); c->p.sigma = c->sigma.data(); c->p.mass = c->mass.data();
     DeviceSystem &s = c->s;
-    Geometry &g = s.g;
-    g.n = p->n;
-    for (int d = 0; d < 3; ++d) {
-        int nc = p->ncell[d] > 0 ? p->ncell[d] : (int)floor((double)p->box[d] / (double)p->rc);
-        if (nc < 1) nc = 1;
-        if ((double)p->box[d] / nc < (double)p->rc) {
-            set_error("md_rl_create: cell edge %g < rc %g on axis %d", p->box[d] / nc, p->rc, d);
-            delete c;
-            return MDRL_ECELL;
+    s.nsys = nsys;
+    s.rc2 = p->rc * p->rc;
+    int atom_off = 0, cell_off = 0;
+    for (int r = 0; r < nsys; ++r) {
+        SysGeom G{};
+        for (int d = 0; d < 3; ++d) {
+            const float L = boxes[3 * r + d];
+            int nc = (nsys == 1 && p->ncell[d] > 0) ? p->ncell[d] : (int)floor((double)L / (double)p->rc);
+            if (nc < 1) nc = 1;
+            if ((double)L / nc < (double)p->rc) {
+                set_error("md_rl_create: system %d: cell edge %g < rc %g on axis %d", r, L / nc, p->rc, d);
+                delete c;
+                return MDRL_ECELL;
+            }
+            G.nc[d] = nc;
+            G.box[d] = L;
+            G.inv_edge[d] = (float)((double)nc / (double)L);
         }
-        g.nc[d] = nc;
-        g.box[d] = p->box[d];
-        g.inv_edge[d] = (float)((double)nc / (double)p->box[d]);
+        G.atom_off = atom_off;
+        G.cell_off = cell_off;
+        G.n = nat[r];
+        atom_off += nat[r];
+        cell_off += G.nc[0] * G.nc[1] * G.nc[2];
+        s.h_geom.push_back(G);
     }
-    g.ncells = g.nc[0] * g.nc[1] * g.nc[2];
-    g.rc2 = p->rc * p->rc;
+    s.n = atom_off;
+    s.ncells = cell_off;
+    c->p.n = s.n;
     s.ntypes = p->ntypes;
     s.flags = p->flags;
     const md_rl_table *t = p->table;
@@ -255,19 +313,21 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     s.coul = p->coulomb_k != 0.0f && !(p->flags & MDRL_NO_COULOMB);
     if (s.coul && !t->has[MDRL_TERM_3] && !(p->flags & MDRL_EVAL_ANALYTIC)) {
         set_error("md_rl_create: Coulomb requested but the table has no r^-3 term");
+        md_rl_table_free(c->own_table);
         delete c;
         return MDRL_EINVAL;
     }
-    int shi = table_segment(g.rc2, t->exp_offset, t->nseg);
+    const int shi = table_segment(s.rc2, t->exp_offset, t->nseg);
     if (shi < 0 && !(p->flags & MDRL_EVAL_ANALYTIC)) {
-        set_error("md_rl_create: rc^2 = %g lies outside the table range", (double)g.rc2);
+        set_error("md_rl_create: rc^2 = %g lies outside the table range", (double)s.rc2);
+        md_rl_table_free(c->own_table);
         delete c;
         return MDRL_EINVAL;
     }
     {
         // LDS window: from seven octaves below rc^2 up to rc^2's own bin
         uint32_t u;
-        memcpy(&u, &g.rc2, 4);
+        memcpy(&u, &s.rc2, 4);
         const int hi_seg = shi < 0 ? 0 : shi;
         const int lo_seg = hi_seg >= 7 ? hi_seg - 7 : 0;
         const int hi_idx = hi_seg * 256 + (int)((u >> 15) & 0xff);
@@ -276,9 +336,18 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
         s.win.stride = s.coul ? 6 : 4;
     }
     c->qscale = s.coul ? sqrt(10.0 * (double)p->coulomb_k * 4.0 * 3.14) : 0.0;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            s.cu_count = cus;
+    }
 
-    const int n = g.n, nt = p->ntypes;
+    const int n = s.n, nt = p->ntypes;
     int st = MDRL_OK;
+    if (!st) st = dalloc(&s.geom, nsys);
+    if (!st) st = dalloc(&s.cell_sys, s.ncells);
+    if (!st) st = dalloc(&s.slot_sys, n);
     if (!st) st = dalloc(&s.pos, n);
     if (!st) st = dalloc(&s.pos2, n);
     if (!st) st = dalloc(&s.vel, n);
@@ -289,9 +358,9 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     if (!st) st = dalloc(&s.orig2, n);
     if (!st) st = dalloc(&s.cell, n);
     if (!st) st = dalloc(&s.perm, n);
-    if (!st) st = dalloc(&s.cell_start, (size_t)g.ncells + 1);
-    if (!st) st = dalloc(&s.cell_next, g.ncells);
-    if (!st) st = dalloc(&s.cell_fill, g.ncells);
+    if (!st) st = dalloc(&s.cell_start, (size_t)s.ncells + 1);
+    if (!st) st = dalloc(&s.cell_next, s.ncells);
+    if (!st) st = dalloc(&s.cell_fill, s.ncells);
     if (!st) st = (p->flags & MDRL_ACC_FP32) ? dalloc(&s.f32, 4 * (size_t)n) : dalloc(&s.fix, 3 * (size_t)n);
     if (!st) st = dalloc(&s.tab, (size_t)t->nent * 6);
     if (!st) st = dalloc(&s.etab, (size_t)t->nent * 6);
@@ -300,14 +369,14 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     if (!st) st = dalloc(&s.hk, nt);
     if (!st) st = dalloc(&s.mass, nt);
     if (!st) st = dalloc(&s.counters, 8);
-    if (!st) st = dalloc(&s.red, 16);
+    if (!st) st = dalloc(&s.cell_red, 10 * (size_t)s.ncells);
+    if (!st) st = dalloc(&s.atom_ke, n);
+    if (!st) st = dalloc(&s.red, (size_t)kRedStride * nsys);
     if (!st) st = dalloc(&c->d_tmp4, n);
     if (st) { md_rl_destroy(c); return st; }
 
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { md_rl_destroy(c); return hip_fail(e, "hipStreamCreate"); }
-    for (auto &ev : c->ev)
-        if ((e = hipEventCreate(&ev)) != hipSuccess) { md_rl_destroy(c); return hip_fail(e, "hipEventCreate"); }
 
     // type-pair multipliers: Lorentz-Berthelot, forces in kJ/mol/nm (10x folded in)
     std::vector<float2> ab(nt * nt), abe(nt * nt);
@@ -328,13 +397,21 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     }
     std::vector<float> tab = interleave(t, MDRL_TERM_14, MDRL_TERM_8, MDRL_TERM_3);
     std::vector<float> etab = interleave(t, MDRL_TERM_12, MDRL_TERM_6, MDRL_TERM_1);
+    std::vector<int> cell_sys(s.ncells), slot_sys(n);
+    for (int r = 0; r < nsys; ++r) {
+        const SysGeom &G = s.h_geom[r];
+        for (int k = 0; k < G.nc[0] * G.nc[1] * G.nc[2]; ++k) cell_sys[G.cell_off + k] = r;
+        for (int k = 0; k < G.n; ++k) slot_sys[G.atom_off + k] = r;
+    }
     unsigned long long zeros[8] = {0};
-    double dz[16] = {0};
 #define UP(dst, src, bytes)                                                              \
     if ((e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess) {          \
         md_rl_destroy(c);                                                                 \
         return hip_fail(e, "hipMemcpy");                                                  \
     }
+    UP(s.geom, s.h_geom.data(), sizeof(SysGeom) * nsys);
+    if (s.ncells) UP(s.cell_sys, cell_sys.data(), sizeof(int) * s.ncells);
+    if (n) UP(s.slot_sys, slot_sys.data(), sizeof(int) * n);
     UP(s.pair_ab, ab.data(), sizeof(float2) * ab.size());
     UP(s.pair_ab_e, abe.data(), sizeof(float2) * abe.size());
     UP(s.hk, hk.data(), sizeof(double) * nt);
@@ -342,114 +419,186 @@ extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
     UP(s.tab, tab.data(), sizeof(float) * tab.size());
     UP(s.etab, etab.data(), sizeof(float) * etab.size());
     UP(s.counters, zeros, sizeof zeros);
-    UP(s.red, dz, sizeof dz);
 #undef UP
+    if ((st = build_tasks(c))) { md_rl_destroy(c); return st; }
+    c->h_x.assign(3 * (size_t)n, 0.0f);
+    c->h_v.assign(3 * (size_t)n, 0.0f);
     c->h_type.assign(n, 0);
     c->h_q.assign(n, 0.0f);
     *out = c;
     return MDRL_OK;
 }
 
-// Uploads host particles (input order) and bins them; computes the forces.
-static int upload_and_bin(md_rl_ctx *c, const float *particle)
+extern "C" int md_rl_create(md_rl_ctx **out, const md_rl_params *p)
+{
+    if (!p) { set_error("md_rl_create: NULL argument"); return MDRL_EINVAL; }
+    if (p->n < 0) { set_error("md_rl_create: n < 0"); return MDRL_EINVAL; }
+    const int32_t n = p->n;
+    return create_common(out, p, 1, &n, p->box);
+}
+
+extern "C" int md_rl_create_batch(md_rl_ctx **out, const md_rl_params *common, int nsys,
+                                  const int32_t *n, const float *box)
+{
+    return create_common(out, common, nsys, n, box);
+}
+
+extern "C" int md_rl_num_systems(const md_rl_ctx *c) { return c ? c->s.nsys : 0; }
+
+extern "C" int md_rl_system_size(const md_rl_ctx *c, int sys, int32_t *n, float box[3], int ncell[3])
+{
+    if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_system_size: bad system"); return MDRL_EINVAL; }
+    const SysGeom &G = c->s.h_geom[sys];
+    if (n) *n = G.n;
+    for (int d = 0; d < 3; ++d) {
+        if (box) box[d] = G.box[d];
+        if (ncell) ncell[d] = G.nc[d];
+    }
+    return MDRL_OK;
+}
+
+// Uploads the staged input of every system, bins it and computes the forces.
+static int upload(md_rl_ctx *c)
 {
     DeviceSystem &s = c->s;
-    const int n = s.g.n;
+    const int n = s.n;
     std::vector<float4> pos(n), vel(n);
     std::vector<int> orig(n);
-    for (int i = 0; i < n; ++i) {
-        float x[3];
-        for (int d = 0; d < 3; ++d) {
-            double v = particle[11 * (size_t)i + d], L = s.g.box[d];
-            if (!isfinite(v)) { set_error("md_rl_set_particles: non-finite position of atom %d", i); return MDRL_ENONFINITE; }
-            float f = (float)v;
-            if (!(f >= 0.0f && f < (float)L)) {
-                f = (float)(v - L * floor(v / L));
-                if (f >= (float)L) f = 0.0f;
+    for (const SysGeom &G : s.h_geom)
+        for (int k = 0; k < G.n; ++k) {
+            const int i = G.atom_off + k;
+            float x[3];
+            for (int d = 0; d < 3; ++d) {
+                double v = c->h_x[3 * (size_t)i + d], L = G.box[d];
+                float f = (float)v;
+                if (!(f >= 0.0f && f < (float)L)) {
+                    f = (float)(v - L * floor(v / L));
+                    if (!(f >= 0.0f && f < (float)L)) f = 0.0f;
+                }
+                x[d] = f;
             }
-            x[d] = f;
+            pos[i] = make_float4(x[0], x[1], x[2], (float)((double)c->h_q[i] * c->qscale));
+            vel[i] = make_float4(c->h_v[3 * (size_t)i], c->h_v[3 * (size_t)i + 1], c->h_v[3 * (size_t)i + 2], 0.0f);
+            orig[i] = i;
         }
-        pos[i] = make_float4(x[0], x[1], x[2], (float)((double)c->h_q[i] * c->qscale));
-        vel[i] = make_float4(particle[11 * (size_t)i + 3], particle[11 * (size_t)i + 4],
-                             particle[11 * (size_t)i + 5], 0.0f);
-        orig[i] = i;
-    }
     hipError_t e;
-    if ((e = hipMemcpyAsync(s.pos, pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(s.vel, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(s.type, c->h_type.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(s.orig, orig.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    if (n > 0 &&
+        ((e = hipMemcpyAsync(s.pos, pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+         (e = hipMemcpyAsync(s.vel, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+         (e = hipMemcpyAsync(s.type, c->h_type.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+         (e = hipMemcpyAsync(s.orig, orig.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess))
         return hip_fail(e, "hipMemcpyAsync");
-    std::vector<int> zero(s.g.ncells, 0);
-    if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * s.g.ncells, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    std::vector<int> zero(s.ncells, 0);
+    if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * s.ncells, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync");
     int rc;
     if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream))) return rc;
     s.first_step = true;
-    c->have_particles = true;
-    if ((rc = launch_zero_forces(s, c->stream, true)) || (rc = launch_force(s, c->stream))) return rc;
+    drop_graphs(c);
+    if ((rc = launch_zero_forces(s, c->stream)) || (rc = launch_force(s, c->stream))) return rc;
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    // input copies must stay alive until the async copies finished: synchronised above
+    c->have_particles = true;
+    c->dirty = false;
     return MDRL_OK;
 }
 
-// Current state back in input order (positions, stored velocities).
-static int download_state(md_rl_ctx *c, std::vector<float4> &pos, std::vector<float4> &vel,
-                          bool full_step_velocity)
+static int ensure_uploaded(md_rl_ctx *c)
+{
+    if (!c->have_particles && !c->dirty) { set_error("no particles set"); return MDRL_EINVAL; }
+    return c->dirty ? upload(c) : MDRL_OK;
+}
+
+// Device state back into the staging arrays (input order): positions and full-step
+// velocities.  Used before re-staging part of the input after steps.
+static int pull_state(md_rl_ctx *c)
 {
     DeviceSystem &s = c->s;
-    const int n = s.g.n;
+    const int n = s.n;
+    if (n == 0) return MDRL_OK;
     std::vector<float4> p(n), v(n);
     std::vector<int> orig(n);
-    hipError_t e;
     int rc;
-    if (full_step_velocity) {
-        if ((rc = launch_zero_red(s, c->stream)) || (rc = launch_kinetic(s, c->d_tmp4, c->stream))) return rc;
-    }
+    if ((rc = launch_energy(s, c->d_tmp4, c->stream))) return rc;
+    hipError_t e;
     if ((e = hipMemcpyAsync(p.data(), s.pos, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(v.data(), full_step_velocity ? c->d_tmp4 : s.vel, sizeof(float4) * n,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(v.data(), c->d_tmp4, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(orig.data(), s.orig, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
         return hip_fail(e, "read-back");
-    pos.resize(n);
-    vel.resize(n);
-    for (int k = 0; k < n; ++k) { pos[orig[k]] = p[k]; vel[orig[k]] = v[k]; }
+    for (int k = 0; k < n; ++k) {
+        const int i = orig[k];
+        c->h_x[3 * (size_t)i] = p[k].x; c->h_x[3 * (size_t)i + 1] = p[k].y; c->h_x[3 * (size_t)i + 2] = p[k].z;
+        c->h_v[3 * (size_t)i] = v[k].x; c->h_v[3 * (size_t)i + 1] = v[k].y; c->h_v[3 * (size_t)i + 2] = v[k].z;
+    }
     return MDRL_OK;
+}
+
+static int stage_system(md_rl_ctx *c, int sys, const float *particle, const int32_t *type,
+                        const float *charge, bool set_types)
+{
+    if (!c || sys < 0 || sys >= c->s.nsys) { set_error("bad system index %d", sys); return MDRL_EINVAL; }
+    const SysGeom &G = c->s.h_geom[sys];
+    if (c->have_particles && !c->dirty) {
+        int rc = pull_state(c);   // keep the other systems (and unset fields) where they are
+        if (rc) return rc;
+    }
+    if (particle) {
+        for (int k = 0; k < G.n; ++k)
+            for (int d = 0; d < 3; ++d) {
+                const float x = particle[11 * (size_t)k + d], v = particle[11 * (size_t)k + 3 + d];
+                if (!std::isfinite(x) || !std::isfinite(v)) {
+                    set_error("non-finite position or velocity of atom %d of system %d", k, sys);
+                    return MDRL_ENONFINITE;
+                }
+            }
+        for (int k = 0; k < G.n; ++k)
+            for (int d = 0; d < 3; ++d) {
+                c->h_x[3 * (size_t)(G.atom_off + k) + d] = particle[11 * (size_t)k + d];
+                c->h_v[3 * (size_t)(G.atom_off + k) + d] = particle[11 * (size_t)k + 3 + d];
+            }
+    }
+    if (set_types) {
+        for (int k = 0; k < G.n; ++k)
+            if (type[k] < 0 || type[k] >= c->s.ntypes) {
+                set_error("type %d of atom %d out of range", type[k], k);
+                return MDRL_EINVAL;
+            }
+        for (int k = 0; k < G.n; ++k) {
+            c->h_type[G.atom_off + k] = type[k];
+            c->h_q[G.atom_off + k] = charge ? charge[k] : 0.0f;
+        }
+    }
+    c->dirty = true;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_system(md_rl_ctx *c, int sys, const float *particle, const int32_t *type,
+                                const float *charge)
+{
+    if (!particle && !type) { set_error("md_rl_set_system: nothing to set"); return MDRL_EINVAL; }
+    return stage_system(c, sys, particle, type, charge, type != nullptr);
 }
 
 extern "C" int md_rl_set_types(md_rl_ctx *c, const int32_t *type, const float *charge, int n)
 {
-    if (!c || !type || n != c->s.g.n) { set_error("md_rl_set_types: bad argument"); return MDRL_EINVAL; }
-    for (int i = 0; i < n; ++i)
-        if (type[i] < 0 || type[i] >= c->s.ntypes) { set_error("md_rl_set_types: type %d of atom %d out of range", type[i], i); return MDRL_EINVAL; }
-    std::vector<float> p11;
-    if (c->have_particles) {
-        std::vector<float4> pos, vel;
-        int rc = download_state(c, pos, vel, false);
-        if (rc) return rc;
-        p11.assign((size_t)n * 11, 0.0f);
-        for (int i = 0; i < n; ++i) {
-            p11[11 * (size_t)i] = pos[i].x; p11[11 * (size_t)i + 1] = pos[i].y; p11[11 * (size_t)i + 2] = pos[i].z;
-            p11[11 * (size_t)i + 3] = vel[i].x; p11[11 * (size_t)i + 4] = vel[i].y; p11[11 * (size_t)i + 5] = vel[i].z;
-        }
-    }
-    c->h_type.assign(type, type + n);
-    if (charge) c->h_q.assign(charge, charge + n);
-    else c->h_q.assign(n, 0.0f);
-    if (c->have_particles) {
-        bool first = c->s.first_step;
-        int rc = upload_and_bin(c, p11.data());
-        c->s.first_step = first;
-        return rc;
-    }
-    return MDRL_OK;
+    if (!c || !type || c->s.nsys != 1 || n != c->s.n) { set_error("md_rl_set_types: bad argument"); return MDRL_EINVAL; }
+    int rc = stage_system(c, 0, nullptr, type, charge, true);
+    if (rc) return rc;
+    return c->have_particles ? upload(c) : MDRL_OK;
 }
 
 extern "C" int md_rl_set_particles(md_rl_ctx *c, const float *particle, int n)
 {
-    if (!c || !particle || n != c->s.g.n) { set_error("md_rl_set_particles: bad argument"); return MDRL_EINVAL; }
-    return upload_and_bin(c, particle);
+    if (!c || !particle || c->s.nsys != 1 || n != c->s.n) { set_error("md_rl_set_particles: bad argument"); return MDRL_EINVAL; }
+    int rc = stage_system(c, 0, particle, nullptr, nullptr, false);
+    if (rc) return rc;
+    return upload(c);
+}
+
+extern "C" int md_rl_commit(md_rl_ctx *c)
+{
+    if (!c) return MDRL_EINVAL;
+    return c->dirty ? upload(c) : MDRL_OK;
 }
 
 static int check_flags(md_rl_ctx *c)
@@ -494,11 +643,12 @@ static void collect_times(md_rl_ctx *c)
 
 extern "C" int md_rl_force_eval(md_rl_ctx *c)
 {
-    if (!c || !c->have_particles) { set_error("md_rl_force_eval: no particles set"); return MDRL_EINVAL; }
-    int rc;
+    if (!c) return MDRL_EINVAL;
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
     md_rl_ctx::StepEvents *ev = c->timing ? next_events(c) : nullptr;
-    if (ev) { (void)hipEventRecord(ev->s0, c->stream); }
-    if ((rc = launch_zero_forces(c->s, c->stream, true))) return rc;
+    if (ev) (void)hipEventRecord(ev->s0, c->stream);
+    if ((rc = launch_zero_forces(c->s, c->stream))) return rc;
     if (ev) (void)hipEventRecord(ev->f0, c->stream);
     if ((rc = launch_force(c->s, c->stream))) return rc;
     if (ev) { (void)hipEventRecord(ev->f1, c->stream); (void)hipEventRecord(ev->s1, c->stream); }
@@ -508,31 +658,60 @@ extern "C" int md_rl_force_eval(md_rl_ctx *c)
     return check_flags(c);
 }
 
-// One step: Motion_Update on the current forces, rebin (which clears the forces), forces at
-// the new positions.
-static int enqueue_step(md_rl_ctx *c, hipStream_t st)
+// One step: Motion_Update on the current forces, rebin (which clears the forces and the
+// pair counters), forces at the new positions.
+static int enqueue_step(md_rl_ctx *c, hipStream_t st, bool events)
 {
     DeviceSystem &s = c->s;
     int rc;
-    md_rl_ctx::StepEvents *ev = c->timing ? next_events(c) : nullptr;
+    md_rl_ctx::StepEvents *ev = events ? next_events(c) : nullptr;
     if (ev) (void)hipEventRecord(ev->s0, st);
     if ((rc = launch_update(s, c->p.dt, st))) return rc;
     if ((rc = launch_rebin(s, st))) return rc;
-    if ((rc = launch_zero_forces(s, st, false))) return rc;
     if (ev) (void)hipEventRecord(ev->f0, st);
     if ((rc = launch_force(s, st))) return rc;
     if (ev) { (void)hipEventRecord(ev->f1, st); (void)hipEventRecord(ev->s1, st); }
     return MDRL_OK;
 }
 
+// Two steps captured as one graph for the current buffer parity (the rebin swaps the
+// double buffers, so two steps return the pointers to where they started).
+static int capture_pair(md_rl_ctx *c)
+{
+    hipGraph_t g = nullptr;
+    MDRL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c, c->stream, false);
+    if (!rc) rc = enqueue_step(c, c->stream, false);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    c->graph[c->s.parity] = ex;
+    return MDRL_OK;
+}
+
 extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
 {
-    if (!c || !c->have_particles || nsteps < 0) { set_error("md_rl_step: no particles set"); return MDRL_EINVAL; }
+    if (!c || nsteps < 0) { set_error("md_rl_step: bad argument"); return MDRL_EINVAL; }
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
     hipStream_t st = hip_stream ? (hipStream_t)hip_stream : c->stream;
-    for (int k = 0; k < nsteps; ++k) {
-        int rc = enqueue_step(c, st);
-        if (rc) return rc;
+    int k = 0;
+    if (c->use_graph && !c->timing && st == c->stream) {
+        if (c->s.first_step && k < nsteps) {
+            if ((rc = enqueue_step(c, st, false))) return rc;
+            ++k;
+        }
+        for (; k + 2 <= nsteps; k += 2) {
+            if (!c->graph[c->s.parity] && (rc = capture_pair(c))) return rc;
+            MDRL_HIP(hipGraphLaunch(c->graph[c->s.parity], st));
+        }
     }
+    for (; k < nsteps; ++k)
+        if ((rc = enqueue_step(c, st, c->timing))) return rc;
     return MDRL_OK;
 }
 
@@ -554,15 +733,16 @@ extern "C" int md_rl_step(md_rl_ctx *c, int nsteps)
 
 extern "C" int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, float gamma)
 {
-    if (!c || !c->have_particles || iterations < 0 || !(max_disp > 0) || !(gamma > 0)) {
+    if (!c || iterations < 0 || !(max_disp > 0) || !(gamma > 0)) {
         set_error("md_rl_minimize: bad argument");
         return MDRL_EINVAL;
     }
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
     DeviceSystem &s = c->s;
-    int rc;
     for (int k = 0; k < iterations; ++k) {
         if ((rc = launch_minimize(s, gamma, max_disp, c->stream)) || (rc = launch_rebin(s, c->stream)) ||
-            (rc = launch_zero_forces(s, c->stream, false)) || (rc = launch_force(s, c->stream)))
+            (rc = launch_force(s, c->stream)))
             return rc;
     }
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -571,63 +751,117 @@ extern "C" int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, floa
     return check_flags(c);
 }
 
-extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)
+// Slots [atom_off, atom_off + n) of one system: positions, full-step velocities (kinetic
+// kernel), forces and input indices, scattered into the caller's input order.
+static int read_system(md_rl_ctx *c, int sys, float *particle, int64_t *fix)
 {
-    if (!c || !fix || n != c->s.g.n) { set_error("md_rl_get_forces_fixed: bad argument"); return MDRL_EINVAL; }
     DeviceSystem &s = c->s;
+    const SysGeom &G = s.h_geom[sys];
+    const int n = G.n;
+    if (n == 0) return MDRL_OK;
+    std::vector<float4> p(n), v(n);
     std::vector<int> orig(n);
     std::vector<long long> f(3 * (size_t)n);
-    hipError_t e;
+    std::vector<float> g;
+    int rc;
+    if (particle && (rc = launch_energy(s, c->d_tmp4, c->stream))) return rc;
+    hipError_t e = hipSuccess;
+    const size_t o = G.atom_off;
+    if (particle) {
+        if (e == hipSuccess) e = hipMemcpyAsync(p.data(), s.pos + o, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(v.data(), c->d_tmp4 + o, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream);
+    }
     if (s.flags & MDRL_ACC_FP32) {
-        std::vector<float> g(4 * (size_t)n);
-        if ((e = hipMemcpy(g.data(), s.f32, sizeof(float) * 4 * n, hipMemcpyDeviceToHost)) != hipSuccess)
-            return hip_fail(e, "hipMemcpy");
+        g.resize(4 * (size_t)n);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.data(), s.f32 + 4 * o, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, c->stream);
+    } else if (e == hipSuccess) {
+        e = hipMemcpyAsync(f.data(), s.fix + 3 * o, sizeof(long long) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(orig.data(), s.orig + o, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "read-back");
+    if (s.flags & MDRL_ACC_FP32)
         for (int k = 0; k < n; ++k)
             for (int d = 0; d < 3; ++d) f[3 * (size_t)k + d] = llrint((double)g[4 * (size_t)k + d] * kFixScale);
-    } else if ((e = hipMemcpy(f.data(), s.fix, sizeof(long long) * 3 * n, hipMemcpyDeviceToHost)) != hipSuccess) {
-        return hip_fail(e, "hipMemcpy");
+    for (int k = 0; k < n; ++k) {
+        const int i = orig[k] - G.atom_off;
+        if (fix)
+            for (int d = 0; d < 3; ++d) fix[3 * (size_t)i + d] = f[3 * (size_t)k + d];
+        if (particle) {
+            float *q = particle + 11 * (size_t)i;
+            q[0] = p[k].x; q[1] = p[k].y; q[2] = p[k].z;
+            q[3] = v[k].x; q[4] = v[k].y; q[5] = v[k].z;
+            for (int d = 0; d < 3; ++d) q[6 + d] = (float)((double)f[3 * (size_t)k + d] / kFixScale * 0.1);
+            q[9] = 0.0f;
+            q[10] = v[k].w;
+        }
     }
-    if ((e = hipMemcpy(orig.data(), s.orig, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess)
-        return hip_fail(e, "hipMemcpy");
-    for (int k = 0; k < n; ++k)
-        for (int d = 0; d < 3; ++d) fix[3 * (size_t)orig[k] + d] = f[3 * (size_t)k + d];
     return MDRL_OK;
+}
+
+extern "C" int md_rl_get_system(md_rl_ctx *c, int sys, float *particle)
+{
+    if (!c || !particle || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_get_system: bad argument"); return MDRL_EINVAL; }
+    int rc = ensure_uploaded(c);
+    return rc ? rc : read_system(c, sys, particle, nullptr);
+}
+
+extern "C" int md_rl_get_system_forces_fixed(md_rl_ctx *c, int sys, int64_t *fix)
+{
+    if (!c || !fix || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_get_system_forces_fixed: bad argument"); return MDRL_EINVAL; }
+    int rc = ensure_uploaded(c);
+    return rc ? rc : read_system(c, sys, nullptr, fix);
 }
 
 extern "C" int md_rl_get_particles(md_rl_ctx *c, float *particle, int n)
 {
-    if (!c || !particle || n != c->s.g.n) { set_error("md_rl_get_particles: bad argument"); return MDRL_EINVAL; }
-    std::vector<float4> pos, vel;
-    int rc = download_state(c, pos, vel, true);
+    if (!c || !particle || c->s.nsys != 1 || n != c->s.n) { set_error("md_rl_get_particles: bad argument"); return MDRL_EINVAL; }
+    return md_rl_get_system(c, 0, particle);
+}
+
+extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)
+{
+    if (!c || !fix || c->s.nsys != 1 || n != c->s.n) { set_error("md_rl_get_forces_fixed: bad argument"); return MDRL_EINVAL; }
+    return md_rl_get_system_forces_fixed(c, 0, fix);
+}
+
+static int energies(md_rl_ctx *c, std::vector<double> &red)
+{
+    int rc = ensure_uploaded(c);
     if (rc) return rc;
-    std::vector<int64_t> fix(3 * (size_t)n);
-    if ((rc = md_rl_get_forces_fixed(c, fix.data(), n))) return rc;
-    for (int i = 0; i < n; ++i) {
-        float *p = particle + 11 * (size_t)i;
-        p[0] = pos[i].x; p[1] = pos[i].y; p[2] = pos[i].z;
-        p[3] = vel[i].x; p[4] = vel[i].y; p[5] = vel[i].z;
-        for (int d = 0; d < 3; ++d) p[6 + d] = (float)((double)fix[3 * (size_t)i + d] / kFixScale * 0.1);
-        p[9] = 0.0f;
-        p[10] = vel[i].w;
-    }
+    DeviceSystem &s = c->s;
+    if ((rc = launch_energy(s, nullptr, c->stream))) return rc;
+    red.resize((size_t)kRedStride * s.nsys);
+    hipError_t e = hipMemcpyAsync(red.data(), s.red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? MDRL_OK : hip_fail(e, "hipMemcpy");
+}
+
+extern "C" int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke, double virial[9])
+{
+    if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_system_energy: bad argument"); return MDRL_EINVAL; }
+    std::vector<double> red;
+    int rc = energies(c, red);
+    if (rc) return rc;
+    const double *r = red.data() + (size_t)kRedStride * sys;
+    if (ke) *ke = r[0];
+    if (pe) *pe = r[1];
+    if (virial) for (int k = 0; k < 9; ++k) virial[k] = r[2 + k];
     return MDRL_OK;
 }
 
 extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9])
 {
-    if (!c || !c->have_particles) { set_error("md_rl_energy: no particles set"); return MDRL_EINVAL; }
-    DeviceSystem &s = c->s;
-    int rc;
-    if ((rc = launch_zero_red(s, c->stream)) || (rc = launch_kinetic(s, c->d_tmp4, c->stream)) ||
-        (rc = launch_energy(s, c->stream)))
-        return rc;
-    double red[16];
-    hipError_t e = hipMemcpyAsync(red, s.red, sizeof red, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
-    if (ke) *ke = red[0];
-    if (pe) *pe = red[1];
-    if (virial) for (int k = 0; k < 9; ++k) virial[k] = red[2 + k];
+    if (!c) { set_error("md_rl_energy: bad argument"); return MDRL_EINVAL; }
+    std::vector<double> red;
+    int rc = energies(c, red);
+    if (rc) return rc;
+    double t[11] = {0};
+    for (int r = 0; r < c->s.nsys; ++r)
+        for (int k = 0; k < 11; ++k) t[k] += red[(size_t)kRedStride * r + k];
+    if (ke) *ke = t[0];
+    if (pe) *pe = t[1];
+    if (virial) for (int k = 0; k < 9; ++k) virial[k] = t[2 + k];
     return MDRL_OK;
 }
 
@@ -652,6 +886,73 @@ extern "C" int md_rl_set_timing(md_rl_ctx *c, int on)
     if (!c) return MDRL_EINVAL;
     c->timing = on != 0;
     c->tev_used = 0;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_graph(md_rl_ctx *c, int on)
+{
+    if (!c) return MDRL_EINVAL;
+    c->use_graph = on != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_schedule(md_rl_ctx *c, const md_rl_schedule *sc)
+{
+    if (!c || !sc) return MDRL_EINVAL;
+    Schedule s{sc->atoms_per_wave, sc->tile_depth, sc->offset_groups, sc->persistent};
+    if (!schedule_supported(c->s, s)) {
+        set_error("schedule (apw %d, tile %d, groups %d) not available for this system's flags",
+                  s.apw, s.tile, s.groups);
+        return MDRL_EINVAL;
+    }
+    const bool regroup = s.groups != c->s.sched.groups;
+    c->s.sched = s;
+    drop_graphs(c);
+    return regroup ? build_tasks(c) : MDRL_OK;
+}
+
+extern "C" int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *sc)
+{
+    if (!c || !sc) return MDRL_EINVAL;
+    sc->atoms_per_wave = c->s.sched.apw;
+    sc->tile_depth = c->s.sched.tile;
+    sc->offset_groups = c->s.sched.groups;
+    sc->persistent = c->s.sched.persistent;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_autotune(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms)
+{
+    if (!c || reps < 1) return MDRL_EINVAL;
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
+    const bool was_timing = c->timing;
+    c->timing = true;
+    md_rl_schedule cur, win{};
+    md_rl_get_schedule(c, &cur);
+    double win_ms = 1e300;
+    for (int apw : {64, 32, 16})
+        for (int tile : {256, 128})
+            for (int groups : {4, 2, 1})
+                for (int pers : {0, 1}) {
+                    md_rl_schedule t{apw, tile, groups, pers};
+                    if (md_rl_set_schedule(c, &t) != MDRL_OK) continue;
+                    if ((rc = md_rl_force_eval(c))) goto out;   // warm-up
+                    double tot = 0.0;
+                    for (int r = 0; r < reps; ++r) {
+                        if ((rc = md_rl_force_eval(c))) goto out;
+                        tot += c->force_ms;
+                    }
+                    if (tot / reps < win_ms) { win_ms = tot / reps; win = t; }
+                }
+out:
+    c->timing = was_timing;
+    if (rc) { md_rl_set_schedule(c, &cur); return rc; }
+    if ((rc = md_rl_set_schedule(c, &win))) return rc;
+    if ((rc = md_rl_force_eval(c))) return rc;
+    if (best) *best = win;
+    if (best_ms) *best_ms = win_ms;
     return MDRL_OK;
 }
 

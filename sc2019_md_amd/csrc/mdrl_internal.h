@@ -1,5 +1,5 @@
 // mdrl_internal.h -- host-side structures shared by the C ABI (mdrl_api.cpp) and the
-// kernel launchers (mdrl_kernels.hip).  Not installed; not part of the ABI.
+// kernel launchers (mdrl_system.hip, mdrl_kernels.hip).  Not installed; not part of the ABI.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,14 +30,25 @@ int hip_fail(hipError_t e, const char *what);   // records the error, returns MD
 // Fixed-point scale: forces accumulate as int64 multiples of 2^-32 kJ/mol/nm.
 constexpr double kFixScale = 4294967296.0;
 
-// Geometry of one periodic system (one replica).
-struct Geometry {
-    int n = 0;
-    int nc[3] = {1, 1, 1};
-    int ncells = 1;
-    float box[3] = {0, 0, 0};
-    float inv_edge[3] = {0, 0, 0};      // (float)(nc / box), the binning rule
-    float rc2 = 0;
+// One periodic system of a batch (a single system is a batch of one).  Its atoms occupy
+// slots [atom_off, atom_off + n) and its cells [cell_off, cell_off + nc0*nc1*nc2) of the
+// batch-wide arrays; sorting never moves an atom out of its system's range.
+struct SysGeom {
+    float box[3];
+    float inv_edge[3];      // (float)(nc / box): the binning rule
+    int nc[3];
+    int cell_off;
+    int atom_off;
+    int n;
+};
+
+// Force-kernel schedule (autotuned): home atoms per wavefront, neighbour-tile depth,
+// half-shell offset groups per workgroup, persistent work queue.
+struct Schedule {
+    int apw = 64;
+    int tile = 256;
+    int groups = 4;
+    int persistent = 0;
 };
 
 // Table window kept in LDS by the force kernel: entries [lo, lo + n).
@@ -46,16 +57,26 @@ struct TableWindow {
     int stride = 6;                     // floats per entry (6 with r^-3, 4 without)
 };
 
-// Device state of one system.
+constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], -
+
+// Device state of a batch of systems.
 struct DeviceSystem {
-    Geometry g;
+    int nsys = 1;
+    int n = 0;                                // atoms over all systems
+    int ncells = 0;                           // cells over all systems
+    float rc2 = 0.f;
     int ntypes = 1;
     int flags = 0;
     bool coul = false;
-    int cell_cap = 0;
+    std::vector<SysGeom> h_geom;
+    SysGeom *geom = nullptr;                  // [nsys]
+    int *cell_sys = nullptr;                  // [ncells] system of each cell
+    int *slot_sys = nullptr;                  // [n] system of each slot (static)
+    int *tasks = nullptr;                     // force tasks, largest first
+    int ntasks = 0;
     // particle data in cell-sorted order (double-buffered for the rebin gather)
     float4 *pos = nullptr, *pos2 = nullptr;   // x, y, z, scaled charge
-    float4 *vel = nullptr, *vel2 = nullptr;   // vx, vy, vz, kinetic energy
+    float4 *vel = nullptr, *vel2 = nullptr;   // vx, vy, vz (half step), -
     int *type = nullptr, *type2 = nullptr;
     int *orig = nullptr, *orig2 = nullptr;    // input index of each slot
     int *cell = nullptr;                      // cell of each slot after the update
@@ -68,26 +89,32 @@ struct DeviceSystem {
     float *tab = nullptr;                     // force table, 6 floats / entry
     float *etab = nullptr;                    // energy table (12, 6, 1), 6 floats / entry
     float2 *pair_ab = nullptr;                // ntypes^2 (A, B) for the kernel's mode
-    float2 *pair_ab_e = nullptr;              // ntypes^2 (A, B) energy multipliers
+    float2 *pair_ab_e = nullptr;              // ntypes^2 (eps sig^12, eps sig^6)
     double *hk = nullptr;                     // ntypes: half-kick factor
     double *mass = nullptr;                   // ntypes
-    unsigned long long *counters = nullptr;   // [0] pairs [1] candidates [2] flags [3] max cell
-    double *red = nullptr;                    // [0] KE [1] PE [2..10] virial
+    unsigned long long *counters = nullptr;   // [0] pairs [1] candidates [2] flags [3] max cell [4] work
+    double *cell_red = nullptr;               // ncells x 10: PE + virial partials per home cell
+    double *atom_ke = nullptr;                // n: kinetic energy per slot
+    double *red = nullptr;                    // nsys x kRedStride
     int exp_offset = 121, nseg = 14;
     TableWindow win;
+    Schedule sched;
+    int cu_count = 256;
     bool first_step = true;
+    int parity = 0;                           // ping-pong state of the double buffers
 };
 
-// Kernel launchers (mdrl_kernels.hip).
+// Kernel launchers (mdrl_system.hip).
 int launch_bin_initial(DeviceSystem &s, hipStream_t st);   // cell ids of the current slots
 int launch_rebin(DeviceSystem &s, hipStream_t st);         // scan, scatter, sort, gather
 int launch_force(DeviceSystem &s, hipStream_t st);
 int launch_update(DeviceSystem &s, float dt, hipStream_t st);
 int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
-int launch_energy(DeviceSystem &s, hipStream_t st);
-int launch_zero_forces(DeviceSystem &s, hipStream_t st, bool forces);  // pair counters (+ forces)
-int launch_zero_red(DeviceSystem &s, hipStream_t st);     // FP64 reductions
-int launch_kinetic(DeviceSystem &s, float4 *vout, hipStream_t st);
+int launch_zero_forces(DeviceSystem &s, hipStream_t st);   // forces and pair counters
+int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st);   // red[] per system
+bool schedule_supported(const DeviceSystem &s, const Schedule &sc);
+
+// Kernel launchers (mdrl_kernels.hip).
 int launch_eval_pairs(const float *d_tab, int exp_offset, int nseg, const float *ref,
                       const float *nbr, const float *pa, const float *pb, const float *pqq,
                       float *fs, float *fxyz, int64_t n, float rc2, int flags, hipStream_t st);

@@ -1,0 +1,795 @@
+// mdrl_system.hip -- gfx950 kernels of the half-shell system path: force evaluation,
+// binning (counting sort), Motion_Update, pre-relaxation and FP64 energy reductions, for a
+// batch of independent periodic systems (a single system is a batch of one).
+//
+// Reference paths: D/ = Range_Limited_demo/SourceCode/, R/ = Range_Limited/,
+// M/ = D/MatlabScripts/, O/ = D/OpenCL_HDL_Lib/.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mdrl_device.h"
+
+namespace mdrl {
+
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+// ---------------------------------------------------------------------------------
+// k_rl_halfshell: one workgroup of 8 waves per task = (home cell, group of half-shell
+// offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
+// first) from a counter.
+//
+// The home cell's atoms are staged in LDS (tile of TI = 256) and each neighbour cell of
+// the group (pre-shifted by its periodic image) as a tile of TJ.  A lane holds one home
+// atom in each of up to four chunks of APW atoms (APW = atoms per wavefront: with APW < 64,
+// 64/APW lanes share a home atom and split its neighbours).  At step s, lane l tests the
+// neighbour (l mod APW) + (l / APW) + s*64/APW (mod m) against all its home atoms, so within
+// a step both indices of the queued pairs are distinct and one LDS read feeds every chunk;
+// steps are strided over the 8 waves.  The r2 <= rc2 test of D/Filter_Logic.v:234 is a
+// ballot; passing pairs are compacted into a per-wave LDS ring (the filter buffers and
+// arbiter of D/Filter_Bank.v:548-606) and, whenever 64 are queued, all 64 lanes evaluate
+// them densely: table force from an LDS window, then the reference particle gets +F and the
+// neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed point
+// (exact, order-free) or FP32.  Neighbour forces are flushed to HBM with one atomic per
+// touched particle and component after each neighbour tile
+// (D/Force_Write_Back_Controller.v:370-429), home forces after the group's last offset
+// (D/Partial_Force_Acc.v:65-129).
+// ---------------------------------------------------------------------------------
+
+constexpr int TI = 256, QCAP = 128, NWAVE = 8, NTHR = 512, NCHUNK = 4;
+
+struct ForceArgs {
+    const float4 *pos;
+    const int *type;
+    const int *cell_start;
+    const SysGeom *geom;
+    const int *cell_sys;
+    const int *tasks;         // task order (persistent mode); NULL: task = blockIdx
+    int ntasks;
+    float rc2;
+    const float *tab;
+    int off, nseg, win_lo, nwin;
+    const float2 *pair_ab;
+    int ntypes;
+    long long *fix;
+    float *f32;
+    unsigned long long *counters;
+    int ngroups;
+    int gstart[8];
+};
+
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ>
+__global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
+{
+    constexpr int STR = COUL ? 6 : 4;
+    constexpr int P = 64 / APW;
+    extern __shared__ float s_tab[];
+    __shared__ float4 s_i[TI];
+    __shared__ float4 s_j[TJ];
+    __shared__ int s_it[TI];
+    __shared__ int s_jt[TJ];
+    __shared__ unsigned long long s_iacc[3 * TI];
+    __shared__ unsigned long long s_jacc[3 * TJ];
+    __shared__ unsigned s_q[NWAVE][QCAP];
+    __shared__ float2 s_ab[64];
+    __shared__ int s_task;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
+    const int il = lane % APW, ph = lane / APW;
+
+    // table window and type-pair multipliers
+    if (!ANA) {
+        const float *g = a.tab + (size_t)a.win_lo * 6;
+        for (int e = tid; e < a.nwin; e += NTHR) {
+            const float *p = g + (size_t)e * 6;
+            float *q = s_tab + e * STR;
+            q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[3];
+            if (STR == 6) { q[4] = p[4]; q[5] = p[5]; }
+        }
+    }
+    for (int e = tid; e < a.ntypes * a.ntypes; e += NTHR) s_ab[e] = a.pair_ab[e];
+
+    float *s_iacc_f = reinterpret_cast<float *>(s_iacc);
+    float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
+    unsigned *q = s_q[wave];
+    unsigned flag = 0;
+    unsigned long long npairs = 0, ncand = 0;
+    const float rc2 = a.rc2;
+
+    for (int it = 0;; ++it) {
+        int task;
+        if (a.tasks) {
+            __syncthreads();
+            if (tid == 0) s_task = (int)atomicAdd(reinterpret_cast<unsigned *>(&a.counters[4]), 1u);
+            __syncthreads();
+            const int t = s_task;
+            if (t >= a.ntasks) break;
+            task = a.tasks[t];
+        } else {
+            if (it > 0) break;
+            task = blockIdx.x;
+        }
+        const int h = task / a.ngroups, grp = task - h * a.ngroups;
+        const SysGeom &G = a.geom[a.cell_sys[h]];
+        const int nc0 = G.nc[0], nc1 = G.nc[1], nc2 = G.nc[2];
+        const int hl = h - G.cell_off;
+        const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
+        const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
+
+        for (int itile = 0; itile < hcount; itile += TI) {
+            const int mi = min(TI, hcount - itile);
+            __syncthreads();
+            for (int k = tid; k < mi; k += NTHR) {
+                s_i[k] = a.pos[hstart + itile + k];
+                s_it[k] = a.type[hstart + itile + k];
+            }
+            for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
+
+            for (int o = a.gstart[grp]; o < a.gstart[grp + 1]; ++o) {
+                const bool self = (o == 0);
+                int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
+                float sx = 0.f, sy = 0.f, sz = 0.f;
+                if (rx < 0) { rx += nc0; sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; sx = G.box[0]; }
+                if (ry < 0) { ry += nc1; sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; sy = G.box[1]; }
+                if (rz < 0) { rz += nc2; sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; sz = G.box[2]; }
+                const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
+                const int cstart = a.cell_start[c], ccount = a.cell_start[c + 1] - cstart;
+
+                for (int jtile = 0; jtile < ccount; jtile += TJ) {
+                    if (self && jtile + TJ <= itile) continue;   // all pairs below the diagonal
+                    const int mj = min(TJ, ccount - jtile);
+                    __syncthreads();
+                    for (int k = tid; k < mj; k += NTHR) {
+                        float4 p = a.pos[cstart + jtile + k];
+                        if (!PAR) { p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; }
+                        s_j[k] = p;
+                        s_jt[k] = a.type[cstart + jtile + k];
+                    }
+                    for (int k = tid; k < 3 * mj; k += NTHR) s_jacc[k] = 0ull;
+                    __syncthreads();
+
+                    auto eval = [&](unsigned e) {
+                        const int ia = (int)(e & 0xffffu), jb = (int)(e >> 16);
+                        const float4 pa = s_i[ia], pb = s_j[jb];
+                        float dx, dy, dz;
+                        const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
+                        const float2 ab = s_ab[s_it[ia] * a.ntypes + s_jt[jb]];
+                        const float qq = pa.w * pb.w;
+                        const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
+                                                                         a.off, a.nseg, a.win_lo, a.nwin);
+                        const float fx = F * dx, fy = F * dy, fz = F * dz;
+                        if (F32) {
+                            atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
+                            atomic_add_wg(&s_iacc_f[3 * ia + 1], fy);
+                            atomic_add_wg(&s_iacc_f[3 * ia + 2], fz);
+                            atomic_add_wg(&s_jacc_f[3 * jb + 0], -fx);
+                            atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
+                            atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
+                        } else {
+                            const long long kx = fix_of(fx, flag), ky = fix_of(fy, flag), kz = fix_of(fz, flag);
+                            atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
+                            atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
+                            atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
+                            atomic_sub_wg(&s_jacc[3 * jb + 0], (unsigned long long)kx);
+                            atomic_sub_wg(&s_jacc[3 * jb + 1], (unsigned long long)ky);
+                            atomic_sub_wg(&s_jacc[3 * jb + 2], (unsigned long long)kz);
+                        }
+                    };
+
+                    unsigned head = 0, tail = 0;
+                    const int bj = il % mj;
+                    const int nsteps = (mj + P - 1) / P;
+                    for (int cg0 = 0; cg0 < mi; cg0 += NCHUNK * APW) {
+                        if (self && itile + cg0 >= jtile + mj) break;   // rest is above the diagonal
+                        const int nch = min(NCHUNK, (mi - cg0 + APW - 1) / APW);
+                        // this lane's home atoms (far away past the tile: r2 = inf never
+                        // passes) and, in the home cell, the neighbour index each pair must
+                        // exceed so that a pair is taken once as (lower, higher)
+                        float4 pi[NCHUNK];
+                        int lim[NCHUNK];
+#pragma unroll
+                        for (int ch = 0; ch < NCHUNK; ++ch) {
+                            const int ia = cg0 + ch * APW + il;
+                            pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+                            lim[ch] = self ? itile + ia - jtile : -1;
+                        }
+                        for (int sft = wave; sft < nsteps; sft += NWAVE) {
+                            const int k = ph + sft * P;
+                            const bool kv = (P == 1) || (k < mj);
+                            int jb = bj + k;
+                            jb = (jb >= mj) ? jb - mj : jb;
+                            jb = kv ? jb : 0;
+                            const float4 pj = s_j[jb];
+#pragma unroll
+                            for (int ch = 0; ch < NCHUNK; ++ch) {
+                                if (ch < nch) {
+                                    float dx, dy, dz;
+                                    const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
+                                    const int ia = cg0 + ch * APW + il;
+                                    const bool pass = kv && (r2 > 0.0f) && (r2 <= rc2) && (jb > lim[ch]);
+                                    const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
+                                    tail += (unsigned)__popcll(m);
+                                    if (tail - head >= 64u) {
+                                        __builtin_amdgcn_wave_barrier();
+                                        eval(q[(head + lane) & (QCAP - 1)]);
+                                        __builtin_amdgcn_wave_barrier();
+                                        head += 64u;
+                                    }
+                                }
+                            }
+                        }
+                        ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
+                    }
+                    npairs += tail;
+                    if (tail != head) {
+                        __builtin_amdgcn_wave_barrier();
+                        if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    __syncthreads();
+                    // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)
+                    if (F32) {
+                        for (int k = tid; k < 3 * mj; k += NTHR) {
+                            float v = s_jacc_f[k];
+                            if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)(cstart + jtile + k / 3) + k % 3], v);
+                        }
+                    } else {
+                        long long *dst = a.fix + 3 * (size_t)(cstart + jtile);
+                        for (int k = tid; k < 3 * mj; k += NTHR) {
+                            unsigned long long v = s_jacc[k];
+                            if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(dst + k), v);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // flush reference-particle forces (D/Partial_Force_Acc.v:65-129)
+            if (F32) {
+                for (int k = tid; k < 3 * mi; k += NTHR) {
+                    float v = s_iacc_f[k];
+                    if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)(hstart + itile + k / 3) + k % 3], v);
+                }
+            } else {
+                long long *dst = a.fix + 3 * (size_t)(hstart + itile);
+                for (int k = tid; k < 3 * mi; k += NTHR) {
+                    unsigned long long v = s_iacc[k];
+                    if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(dst + k), v);
+                }
+            }
+        }
+    }
+    // per-wave statistics (npairs/ncand are wave-uniform)
+    if (lane == 0) {
+        atomic_add_agent(&a.counters[0], npairs);
+        atomic_add_agent(&a.counters[1], ncand * 64ull);
+    }
+    if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Offset groups: the home cell is half the work of a full neighbour cell.
+static void offset_groups(int ngroups, int *gstart)
+{
+    static const int g1[2] = {0, 14};
+    static const int g2[3] = {0, 7, 14};
+    static const int g4[5] = {0, 4, 8, 11, 14};
+    const int *g = ngroups == 1 ? g1 : ngroups == 2 ? g2 : g4;
+    for (int k = 0; k <= ngroups; ++k) gstart[k] = g[k];
+}
+
+bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
+{
+    const bool fast = !(s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC));
+    const bool groups_ok = sc.groups == 1 || sc.groups == 2 || sc.groups == 4;
+    if (!groups_ok) return false;
+    if (sc.apw == 64 && sc.tile == 256) return true;
+    return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
+}
+
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ>
+static void launch_rl(const ForceArgs &a, dim3 grid, size_t lds, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ>), grid, dim3(NTHR), lds, st, a);
+}
+
+template <bool COUL>
+static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, size_t lds, hipStream_t st)
+{
+    if (sc.tile == 128) {
+        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 128>(a, grid, lds, st);
+        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 128>(a, grid, lds, st);
+        else launch_rl<COUL, false, false, false, 64, 128>(a, grid, lds, st);
+    } else {
+        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 256>(a, grid, lds, st);
+        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 256>(a, grid, lds, st);
+        else launch_rl<COUL, false, false, false, 64, 256>(a, grid, lds, st);
+    }
+}
+
+int launch_force(DeviceSystem &s, hipStream_t st)
+{
+    if (s.n == 0) return MDRL_OK;
+    ForceArgs a;
+    a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
+    a.geom = s.geom; a.cell_sys = s.cell_sys;
+    a.tasks = s.sched.persistent ? s.tasks : nullptr;
+    a.rc2 = s.rc2;
+    a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
+    a.win_lo = s.win.lo; a.nwin = s.win.n;
+    a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
+    a.fix = s.fix; a.f32 = s.f32; a.counters = s.counters;
+    a.ngroups = s.sched.groups;
+    offset_groups(a.ngroups, a.gstart);
+    a.ntasks = s.ncells * a.ngroups;
+    const bool f32 = s.flags & MDRL_ACC_FP32, par = s.flags & MDRL_EVAL_PARITY,
+               ana = s.flags & MDRL_EVAL_ANALYTIC, coul = s.coul;
+    const int str = coul ? 6 : 4;
+    const size_t lds = ana ? 0 : (size_t)s.win.n * str * sizeof(float);
+    const int grid_n = s.sched.persistent ? std::min(a.ntasks, 2 * s.cu_count) : a.ntasks;
+    const dim3 grid(grid_n);
+    const Schedule &sc = s.sched;
+    if (ana) {
+        if (coul) { if (f32) launch_rl<true, true, false, true, 64, 256>(a, grid, lds, st); else launch_rl<true, false, false, true, 64, 256>(a, grid, lds, st); }
+        else { if (f32) launch_rl<false, true, false, true, 64, 256>(a, grid, lds, st); else launch_rl<false, false, false, true, 64, 256>(a, grid, lds, st); }
+    } else if (par) {
+        if (coul) { if (f32) launch_rl<true, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<true, false, true, false, 64, 256>(a, grid, lds, st); }
+        else { if (f32) launch_rl<false, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<false, false, true, false, 64, 256>(a, grid, lds, st); }
+    } else if (f32) {
+        if (coul) launch_rl<true, true, false, false, 64, 256>(a, grid, lds, st);
+        else launch_rl<false, true, false, false, 64, 256>(a, grid, lds, st);
+    } else {
+        if (coul) launch_rl_fast<true>(a, sc, grid, lds, st);
+        else launch_rl_fast<false>(a, sc, grid, lds, st);
+    }
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// binning: counting sort by cell, made stable by a per-cell rank on the input index
+// (M/Simulation_Verification_LJArgon.m:143-175 keeps atoms in input order per cell).
+// ---------------------------------------------------------------------------------
+
+__device__ __forceinline__ int global_cell(const SysGeom &G, float x, float y, float z)
+{
+    return G.cell_off + (cell_of(x, G.inv_edge[0], G.nc[0]) * G.nc[1] + cell_of(y, G.inv_edge[1], G.nc[1])) * G.nc[2] +
+           cell_of(z, G.inv_edge[2], G.nc[2]);
+}
+
+__global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const int *slot_sys,
+                         int *cell, int *count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (s < n) {
+        const float4 p = pos[s];
+        c = global_cell(geom[slot_sys[s]], p.x, p.y, p.z);
+        cell[s] = c;
+    }
+    run_add(count, c, s < n);
+}
+
+// Exclusive scan of the next-step counts into cell_start; clears counts and fill
+// pointers; records the largest cell; clears the pair counters and the work counter of
+// the force evaluation that follows.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *start, int *fill,
+                                               unsigned long long *counters)
+{
+    __shared__ int s_sum[1024];
+    __shared__ int s_max[1024];
+    const int t = threadIdx.x;
+    const int per = (ncells + 1023) / 1024;
+    const int b = t * per, e = min(ncells, b + per);
+    int sum = 0, mx = 0;
+    for (int c = b; c < e; ++c) { int v = count[c]; sum += v; mx = max(mx, v); }
+    s_sum[t] = sum;
+    s_max[t] = mx;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? s_sum[t - off] : 0;
+        int m = (t >= off) ? s_max[t - off] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+        s_max[t] = max(s_max[t], m);
+        __syncthreads();
+    }
+    int run = s_sum[t] - sum;
+    for (int c = b; c < e; ++c) {
+        int v = count[c];
+        start[c] = run;
+        run += v;
+        count[c] = 0;
+        fill[c] = 0;
+    }
+    if (t == 1023) {
+        start[ncells] = s_sum[1023];
+        counters[3] = (unsigned long long)s_max[1023];
+        counters[0] = 0ull;
+        counters[1] = 0ull;
+        counters[4] = 0ull;
+    }
+}
+
+__global__ void k_scatter(int n, const int *cell, const int *start, int *fill, int *perm)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (s < n) ? cell[s] : 0;
+    const int k = run_add(fill, c, s < n);
+    if (s < n) perm[start[c] + k] = s;
+}
+
+// Orders each cell's slots by input index.  Cells larger than the LDS capacity keep
+// the scatter order and raise MDRL_F_CELL_OVERFLOW (forces stay correct in int64 mode).
+constexpr int SORT_CAP = 2048;
+__global__ __launch_bounds__(256) void k_cell_sort(const int *start, int *perm, const int *orig,
+                                                   unsigned long long *counters)
+{
+    __shared__ int s_v[SORT_CAP];
+    __shared__ int s_k[SORT_CAP];
+    const int c = blockIdx.x;
+    const int b = start[c], m = start[c + 1] - b;
+    if (m <= 1) return;
+    if (m > SORT_CAP) {
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_or(&counters[2], 4ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        int v = perm[b + k];
+        s_v[k] = v;
+        s_k[k] = orig[v];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        int key = s_k[k], r = 0;
+        for (int u = 0; u < m; ++u) r += (s_k[u] < key);
+        perm[b + r] = s_v[k];
+    }
+}
+
+__global__ void k_gather(int n, const int *perm, const float4 *pos, const float4 *vel,
+                         const int *type, const int *orig, float4 *pos2, float4 *vel2,
+                         int *type2, int *orig2, long long *fix, float *f32)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int o = perm[s];
+    pos2[s] = pos[o];
+    vel2[s] = vel[o];
+    type2[s] = type[o];
+    orig2[s] = orig[o];
+    if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
+    if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int launch_bin_initial(DeviceSystem &s, hipStream_t st)
+{
+    if (s.n > 0)
+        hipLaunchKernelGGL(k_assign, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos, s.geom,
+                           s.slot_sys, s.cell, s.cell_next);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+int launch_rebin(DeviceSystem &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.ncells, s.cell_next, s.cell_start,
+                       s.cell_fill, s.counters);
+    if (s.n > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.cell,
+                           s.cell_start, s.cell_fill, s.perm);
+        hipLaunchKernelGGL(k_cell_sort, dim3(s.ncells), dim3(256), 0, st, s.cell_start, s.perm,
+                           s.orig, s.counters);
+        hipLaunchKernelGGL(k_gather, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.perm, s.pos,
+                           s.vel, s.type, s.orig, s.pos2, s.vel2, s.type2, s.orig2,
+                           (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix,
+                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr);
+    }
+    MDRL_HIP(hipGetLastError());
+    std::swap(s.pos, s.pos2);
+    std::swap(s.vel, s.vel2);
+    std::swap(s.type, s.type2);
+    std::swap(s.orig, s.orig2);
+    s.parity ^= 1;
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// k_update: fused Motion_Update (D/Motion_Update.v:685-805) as velocity Verlet:
+//   dv  = (float)(fix * hk[type])            hk = 2^-32 * 1e-5 / m * dt/2
+//   v   = vh + dv                            (full-step velocity; v = vh on the first step)
+//   vh' = v + dv;  x' = wrap(x + vh'*dt);  next cell of x' counted for the rebin.
+// MDRL_KICK_DRIFT gives the reference's kick-then-drift: v' = v + (dv + dv).
+// ---------------------------------------------------------------------------------
+
+// Periodic wrap into [0, L): one add or subtract of L (the reference's wrap, C model
+// :175-192), then a floor-based wrap for the rare atom that moved more than a box
+// length; a non-finite coordinate becomes 0 (the caller has flagged it already).
+__device__ __forceinline__ float wrap1(float x, float L)
+{
+    if (x < 0.0f) x = x + L;
+    if (x >= L) x = x - L;
+    if (!(x >= 0.0f && x < L)) {
+        x = x - L * floorf(x / L);
+        if (!(x >= 0.0f && x < L)) x = 0.0f;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel, const int *type,
+                                                const long long *fix, const float *f32,
+                                                const double *hk, float dt, const SysGeom *geom,
+                                                const int *slot_sys, int first, int kick_drift,
+                                                int *cell, int *count, unsigned long long *counters)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned flag = 0;
+    int c = 0;
+    if (s < n) {
+        const SysGeom &G = geom[slot_sys[s]];
+        const int t = type[s];
+        const double k = hk[t];
+        float f3[3];
+        if (f32) {
+            float4 f = reinterpret_cast<const float4 *>(f32)[s];
+            f3[0] = (float)((double)f.x * 4294967296.0 * k);
+            f3[1] = (float)((double)f.y * 4294967296.0 * k);
+            f3[2] = (float)((double)f.z * 4294967296.0 * k);
+        } else {
+            f3[0] = (float)((double)fix[3 * (size_t)s + 0] * k);
+            f3[1] = (float)((double)fix[3 * (size_t)s + 1] * k);
+            f3[2] = (float)((double)fix[3 * (size_t)s + 2] * k);
+        }
+        float4 p = pos[s], v = vel[s];
+        float vi[3] = {v.x, v.y, v.z}, xi[3] = {p.x, p.y, p.z};
+        for (int d = 0; d < 3; ++d) {
+            float dv = f3[d];
+            float vn;
+            if (kick_drift) {
+                float dv2 = dv + dv;
+                vn = vi[d] + dv2;
+            } else {
+                float vf = first ? vi[d] : vi[d] + dv;
+                vn = vf + dv;
+            }
+            float step = vn * dt;
+            float x = xi[d] + step;
+            if (!isfinite(x) || !isfinite(vn)) flag |= 2u;
+            xi[d] = wrap1(x, G.box[d]);
+            vi[d] = vn;
+        }
+        pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
+        vel[s] = make_float4(vi[0], vi[1], vi[2], 0.0f);
+        c = global_cell(G, xi[0], xi[1], xi[2]);
+        cell[s] = c;
+    }
+    run_add(count, c, s < n);
+    if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int launch_update(DeviceSystem &s, float dt, hipStream_t st)
+{
+    if (s.n == 0) return MDRL_OK;
+    const bool f32 = s.flags & MDRL_ACC_FP32;
+    hipLaunchKernelGGL(k_update, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos, s.vel, s.type,
+                       f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.hk, dt, s.geom, s.slot_sys,
+                       s.first_step ? 1 : 0, (s.flags & MDRL_KICK_DRIFT) ? 1 : 0, s.cell,
+                       s.cell_next, s.counters);
+    MDRL_HIP(hipGetLastError());
+    s.first_step = false;
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// k_minimize: capped steepest descent, the untimed pre-relaxation of lattice systems
+// whose jitter leaves overlapping atoms.  Per atom: F in kJ/mol/nm (int64 * 2^-32),
+// s = min(gamma, dmax / |F|) in double, x' = wrap(x + (float)(F * s)).
+// ---------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_minimize(int n, float4 *pos, const long long *fix,
+                                                  const float *f32, double gamma, double dmax,
+                                                  const SysGeom *geom, const int *slot_sys,
+                                                  int *cell, int *count,
+                                                  unsigned long long *counters)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned flag = 0;
+    int c = 0;
+    if (s < n) {
+        const SysGeom &G = geom[slot_sys[s]];
+        double f[3];
+        for (int d = 0; d < 3; ++d)
+            f[d] = f32 ? (double)f32[4 * (size_t)s + d]
+                       : (double)fix[3 * (size_t)s + d] * (1.0 / 4294967296.0);
+        const double fm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double sc = (fm * gamma > dmax) ? dmax / fm : gamma;
+        float4 p = pos[s];
+        float xi[3] = {p.x, p.y, p.z};
+        for (int d = 0; d < 3; ++d) {
+            float step = (float)(f[d] * sc);
+            float x = xi[d] + step;
+            if (!isfinite(x)) flag |= 2u;
+            xi[d] = wrap1(x, G.box[d]);
+        }
+        pos[s] = make_float4(xi[0], xi[1], xi[2], p.w);
+        c = global_cell(G, xi[0], xi[1], xi[2]);
+        cell[s] = c;
+    }
+    run_add(count, c, s < n);
+    if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st)
+{
+    if (s.n == 0) return MDRL_OK;
+    const bool f32 = s.flags & MDRL_ACC_FP32;
+    hipLaunchKernelGGL(k_minimize, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos,
+                       f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, gamma, dmax, s.geom,
+                       s.slot_sys, s.cell, s.cell_next, s.counters);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Energies.  k_energy: potential energy and virial of each home cell from the c*_{12,6,1}
+// tables (M/FullScale_Simulation_LJArgon.m:938-1010), half shell, written per cell (no
+// atomics).  k_kinetic: full-step velocity v = vh + dv(F) and 0.5 m v^2 per slot.
+// k_sys_reduce: one workgroup per system sums its cells and slots in a fixed tree order,
+// so the FP64 energies are bit-identical run to run.  Not per step: on request.
+// ---------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_energy(ForceArgs a, const float *etab, const float2 *ab_e,
+                                                double *cell_red)
+{
+    __shared__ double s_r[10][256];
+    const int h = blockIdx.x;
+    const SysGeom &G = a.geom[a.cell_sys[h]];
+    const int nc0 = G.nc[0], nc1 = G.nc[1], nc2 = G.nc[2];
+    const int hl = h - G.cell_off;
+    const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
+    const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
+    double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int ia = threadIdx.x; ia < hcount; ia += blockDim.x) {
+        const float4 pi = a.pos[hstart + ia];
+        const int ti = a.type[hstart + ia];
+        for (int o = 0; o < 14; ++o) {
+            int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            if (rx < 0) { rx += nc0; sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; sx = G.box[0]; }
+            if (ry < 0) { ry += nc1; sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; sy = G.box[1]; }
+            if (rz < 0) { rz += nc2; sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; sz = G.box[2]; }
+            const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
+            const int cs = a.cell_start[c], cc = a.cell_start[c + 1] - cs;
+            for (int jb = (o == 0 ? ia + 1 : 0); jb < cc; ++jb) {
+                float4 pj = a.pos[cs + jb];
+                pj.x = pj.x + sx; pj.y = pj.y + sy; pj.z = pj.z + sz;
+                float dx, dy, dz;
+                float r2 = pair_geom<false>(pi, pj, 0.f, 0.f, 0.f, dx, dy, dz);
+                if (!(r2 > 0.0f && r2 <= a.rc2)) continue;
+                const int tj = a.type[cs + jb];
+                // ab_e = (eps sig^12, eps sig^6): E = A t12 - B t6, F = A t14 - B t8 (unit
+                // tables 4 r^-12, 4 r^-6, 48 r^-14, 24 r^-8); the scaled charges carry 10x.
+                const float2 abe = ab_e[ti * a.ntypes + tj];
+                const int idx = lut_seg(r2, a.off, a.nseg) * 256 + lut_bin(r2);
+                const float *pe = etab + (size_t)idx * 6;
+                const float *pf = a.tab + (size_t)idx * 6;
+                double t12 = (double)pe[1] * r2 + pe[0], t6 = (double)pe[3] * r2 + pe[2];
+                double E = abe.x * t12 - abe.y * t6;
+                double t14 = (double)pf[1] * r2 + pf[0], t8 = (double)pf[3] * r2 + pf[2];
+                double F = abe.x * t14 - abe.y * t8;
+                if (pi.w != 0.0f && pj.w != 0.0f) {
+                    double qq = (double)pi.w * pj.w;
+                    E += qq * ((double)pe[5] * r2 + pe[4]) * 0.1;
+                    F += qq * ((double)pf[5] * r2 + pf[4]) * 0.1;
+                }
+                e += E;
+                const double d3[3] = {dx, dy, dz};
+                for (int u = 0; u < 3; ++u)
+                    for (int v = 0; v < 3; ++v) w[3 * u + v] += F * d3[u] * d3[v];
+            }
+        }
+    }
+    s_r[0][threadIdx.x] = e;
+    for (int k = 0; k < 9; ++k) s_r[k + 1][threadIdx.x] = w[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 10; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) cell_red[10 * (size_t)h + threadIdx.x] = s_r[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_kinetic(int n, const float4 *vel, const int *type,
+                                                 const long long *fix, const float *f32,
+                                                 const double *hk, const double *mass,
+                                                 int first, float4 *vout, double *atom_ke)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int t = type[s];
+    const double k = hk[t];
+    float4 v = vel[s];
+    float vi[3] = {v.x, v.y, v.z};
+    double vv = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        float dv = f32 ? (float)((double)f32[4 * (size_t)s + d] * 4294967296.0 * k)
+                       : (float)((double)fix[3 * (size_t)s + d] * k);
+        float vf = first ? vi[d] : vi[d] + dv;
+        vi[d] = vf;
+        vv += (double)vf * (double)vf;
+    }
+    const double ke = 0.5 * mass[t] * vv * 1.0e4;
+    atom_ke[s] = ke;
+    if (vout) vout[s] = make_float4(vi[0], vi[1], vi[2], (float)ke);
+}
+
+__global__ __launch_bounds__(256) void k_sys_reduce(const SysGeom *geom, const double *cell_red,
+                                                    const double *atom_ke, double *red)
+{
+    __shared__ double s_r[11][256];
+    const SysGeom &G = geom[blockIdx.x];
+    const int ncell = G.nc[0] * G.nc[1] * G.nc[2];
+    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < G.n; k += blockDim.x) acc[0] += atom_ke[G.atom_off + k];
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x)
+        for (int k = 0; k < 10; ++k) acc[1 + k] += cell_red[10 * (size_t)(G.cell_off + c) + k];
+    for (int k = 0; k < 11; ++k) s_r[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 11; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 11) red[kRedStride * (size_t)blockIdx.x + threadIdx.x] = s_r[threadIdx.x][0];
+}
+
+int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
+{
+    if (s.n > 0) {
+        const bool f32 = s.flags & MDRL_ACC_FP32;
+        hipLaunchKernelGGL(k_kinetic, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.vel, s.type,
+                           f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.hk, s.mass,
+                           s.first_step ? 1 : 0, vout, s.atom_ke);
+        ForceArgs a{};
+        a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
+        a.geom = s.geom; a.cell_sys = s.cell_sys;
+        a.rc2 = s.rc2;
+        a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
+        a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
+        hipLaunchKernelGGL(k_energy, dim3(s.ncells), dim3(256), 0, st, a, s.etab, s.pair_ab_e,
+                           s.cell_red);
+    }
+    hipLaunchKernelGGL(k_sys_reduce, dim3(s.nsys), dim3(256), 0, st, s.geom, s.cell_red, s.atom_ke,
+                       s.red);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// fills (kernels instead of hipMemsetAsync, so a captured step stays valid)
+// ---------------------------------------------------------------------------------
+
+__global__ void k_zero_forces(int n, long long *fix, float *f32, unsigned long long *counters)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) { counters[0] = 0ull; counters[1] = 0ull; counters[4] = 0ull; }
+    if (s >= n) return;
+    if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
+    if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int launch_zero_forces(DeviceSystem &s, hipStream_t st)
+{
+    const bool f32 = s.flags & MDRL_ACC_FP32;
+    hipLaunchKernelGGL(k_zero_forces, dim3(nblk(std::max(s.n, 1), 256)), dim3(256), 0, st, s.n,
+                       f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.counters);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+}  // namespace mdrl

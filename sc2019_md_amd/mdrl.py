@@ -59,6 +59,15 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("atoms_per_wave", ctypes.c_int), ("tile_depth", ctypes.c_int),
+                ("offset_groups", ctypes.c_int), ("persistent", ctypes.c_int)]
+
+    def __repr__(self):
+        return "Schedule(apw=%d, tile=%d, groups=%d, persistent=%d)" % (
+            self.atoms_per_wave, self.tile_depth, self.offset_groups, self.persistent)
+
+
 class CModelCfg(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int), ("iterations", ctypes.c_int), ("eps", ctypes.c_float),
@@ -115,6 +124,22 @@ def lib():
         L.md_rl_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.md_rl_set_timing.argtypes = [P, I]
         L.md_rl_destroy.argtypes = [P]
+        L.md_rl_create_batch.argtypes = [ctypes.POINTER(P), ctypes.POINTER(Params), I,
+                                         ctypes.POINTER(ctypes.c_int32), F]
+        L.md_rl_num_systems.argtypes = [P]
+        L.md_rl_system_size.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32), F,
+                                        ctypes.POINTER(I)]
+        L.md_rl_set_system.argtypes = [P, I, F, ctypes.POINTER(ctypes.c_int32), F]
+        L.md_rl_commit.argtypes = [P]
+        L.md_rl_get_system.argtypes = [P, I, F]
+        L.md_rl_get_system_forces_fixed.argtypes = [P, I, ctypes.POINTER(ctypes.c_int64)]
+        L.md_rl_system_energy.argtypes = [P, I, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.md_rl_set_graph.argtypes = [P, I]
+        L.md_rl_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+        L.md_rl_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+        L.md_rl_autotune.argtypes = [P, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_double)]
         L.md_rl_cmodel_default.argtypes = [ctypes.POINTER(CModelCfg)]
         L.md_rl_cmodel_default.restype = None
         L.md_rl_cmodel_run.argtypes = [ctypes.POINTER(CModelCfg), F, F]
@@ -329,6 +354,24 @@ class System:
     def set_timing(self, on=True):
         _check(lib().md_rl_set_timing(self.handle, 1 if on else 0))
 
+    def set_graph(self, on=True):
+        _check(lib().md_rl_set_graph(self.handle, 1 if on else 0))
+
+    def set_schedule(self, apw=64, tile=256, groups=4, persistent=0):
+        sc = Schedule(apw, tile, groups, persistent)
+        _check(lib().md_rl_set_schedule(self.handle, ctypes.byref(sc)))
+
+    def schedule(self):
+        sc = Schedule()
+        _check(lib().md_rl_get_schedule(self.handle, ctypes.byref(sc)))
+        return sc
+
+    def autotune(self, reps=3):
+        sc = Schedule()
+        ms = ctypes.c_double()
+        _check(lib().md_rl_autotune(self.handle, reps, ctypes.byref(sc), ctypes.byref(ms)))
+        return sc, ms.value
+
     def close(self):
         if getattr(self, "handle", None):
             lib().md_rl_destroy(self.handle)
@@ -339,6 +382,73 @@ class System:
             self.close()
         except Exception:
             pass
+
+
+class Batch(System):
+    """Independent periodic systems stepped together (md_rl_create_batch)."""
+
+    def __init__(self, sizes, boxes, rc, eps, sigma, mass, dt=1.0, coulomb_k=0.0, flags=0,
+                 table=None):
+        self._eps = _f32(eps)
+        self._sig = _f32(sigma)
+        self._mass = _f32(mass)
+        p = Params()
+        p.rc = rc
+        p.dt = dt
+        p.ntypes = len(self._eps)
+        p.eps, p.sigma, p.mass = _fp(self._eps), _fp(self._sig), _fp(self._mass)
+        p.coulomb_k = coulomb_k
+        p.flags = flags
+        p.table = table.handle if table is not None else None
+        self._table = table
+        self.sizes = np.ascontiguousarray(sizes, np.int32)
+        b = _f32(boxes, (-1, 3))
+        self.nsys = len(self.sizes)
+        self.n = int(self.sizes.sum())
+        self.flags = flags
+        h = ctypes.c_void_p()
+        _check(lib().md_rl_create_batch(ctypes.byref(h), ctypes.byref(p), self.nsys,
+                                        self.sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        _fp(b)))
+        self.handle = h
+
+    @classmethod
+    def from_synthetic(cls, systems, flags=0, table=None):
+        s0 = systems[0]
+        b = cls([s.n for s in systems], [s.box for s in systems], s0.rc, s0.eps, s0.sigma,
+                s0.mass, dt=s0.dt, coulomb_k=s0.coulomb_k, flags=flags, table=table)
+        for k, s in enumerate(systems):
+            b.set_system(k, s.particle, s.type, s.charge if s.coulomb_k else None)
+        b.commit()
+        return b
+
+    def set_system(self, sys, particle=None, types=None, charge=None):
+        n = int(self.sizes[sys])
+        a = None if particle is None else _f32(particle, (n, 11))
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        q = None if charge is None else _f32(charge)
+        _check(lib().md_rl_set_system(self.handle, sys, None if a is None else _fp(a),
+                                      None if t is None else t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      None if q is None else _fp(q)))
+
+    def commit(self):
+        _check(lib().md_rl_commit(self.handle))
+
+    def system_particles(self, sys):
+        out = np.zeros((int(self.sizes[sys]), 11), np.float32)
+        _check(lib().md_rl_get_system(self.handle, sys, _fp(out)))
+        return out
+
+    def system_forces_fixed(self, sys):
+        out = np.zeros((int(self.sizes[sys]), 3), np.int64)
+        _check(lib().md_rl_get_system_forces_fixed(self.handle, sys, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def system_energy(self, sys):
+        pe, ke = ctypes.c_double(), ctypes.c_double()
+        vir = (ctypes.c_double * 9)()
+        _check(lib().md_rl_system_energy(self.handle, sys, ctypes.byref(pe), ctypes.byref(ke), vir))
+        return pe.value, ke.value, np.array(vir[:]).reshape(3, 3)
 
 
 # ---------------------------------------------------------------------------------- C model
