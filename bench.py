@@ -98,6 +98,52 @@ def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s):
     return base, err
 
 
+def batched_vs_plain(nrep, steps, seed, table):
+    """The batched workload: nrep replicas (log-uniform 256..16,384 atoms, 0.1 atoms/A^3,
+    rc 10 A) stepped as one batch (autotuned schedule, hipGraph-captured steps) against the
+    same replicas stepped one context at a time.  Both produce bit-identical int64 forces,
+    so the error is equal by construction."""
+    systems = [mdrl.SyntheticSystem(4, k, seed=seed) for k in range(nrep)]
+    natoms = sum(s.n for s in systems)
+    b = mdrl.Batch.from_synthetic(systems, table=table)
+    sched, tune_ms = b.autotune(reps=2)
+    b.set_graph(True)
+    b.step(2)
+    pairs = b.stats().pair_evals
+    t0 = time.perf_counter()
+    b.step_async(steps)
+    b.sync()
+    tb = (time.perf_counter() - t0) / steps
+    pairs_b = b.stats().pair_evals
+    b.close()
+    plain = [mdrl.System.from_synthetic(s, table=table) for s in systems]
+    for g in plain:
+        g.step(2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for g in plain:
+            g.step_async(1)
+    for g in plain:
+        g.sync()
+    tp = (time.perf_counter() - t0) / steps
+    pairs_p = sum(g.stats().pair_evals for g in plain)
+    for g in plain:
+        g.close()
+    return {
+        "workload": "%d replicas, %d atoms total, log-uniform 256..16,384 atoms each, "
+                    "0.1 atoms/A^3, rc 10 A, dt 1 fs" % (nrep, natoms),
+        "steps": steps,
+        "batched": {"ms_per_step": tb * 1e3, "pair_evals_per_s": pairs_b / tb,
+                    "schedule": sched.as_dict(),
+                    "graph": True},
+        "plain": {"ms_per_step": tp * 1e3, "pair_evals_per_s": pairs_p / tp,
+                  "note": "one context per replica, one step launched per replica in turn"},
+        "pair_evals_per_step": pairs_b,
+        "batched_over_plain": (pairs_b / tb) / (pairs_p / tp),
+        "force_error": "equal (bit-identical int64 forces)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +158,12 @@ def main():
     ap.add_argument("--relax-step", type=float, default=0.02, help="max displacement per iteration (A)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replicas", type=int, default=512, help="batched-vs-plain replicas (0: skip)")
+    ap.add_argument("--replica-steps", type=int, default=20)
+    ap.add_argument("--graph", type=int, default=1, help="capture the timed steps in hipGraphs")
+    ap.add_argument("--dt", type=float, default=None, help="time step in fs (default: the workload's)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run one device copy of this many bytes (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -119,6 +171,8 @@ def main():
         sys.exit(2)
 
     s = mdrl.SyntheticSystem(args.workload, seed=args.seed)
+    if args.dt is not None:
+        s.dt = float(args.dt)
     flags = (mdrl.ACC_FP32 if args.fp32_atomics else 0) | (mdrl.EVAL_ANALYTIC if args.analytic else 0)
     g = mdrl.System.from_synthetic(s, flags=flags)
     fix0 = g.forces_fixed()
@@ -129,19 +183,34 @@ def main():
         g.minimize(relax, args.relax_step, 1e-4)
     pe_relaxed, ke0, _ = g.energy()
 
+    sched, tune_ms = g.autotune(reps=3)
     g.step(args.warmup)
-    g.set_timing(True)
+    # timed region: the steps as a user runs them (graph-captured unless --graph 0)
+    g.set_graph(bool(args.graph))
     t0 = time.perf_counter()
     g.step_async(args.steps)
     g.sync()
     wall = time.perf_counter() - t0
+    pairs_last = g.stats().pair_evals
+    # kernel time: the same number of steps again, each force launch bracketed by HIP
+    # events recorded on the stream it runs on
+    g.set_graph(False)
+    g.set_timing(True)
+    g.step_async(args.steps)
+    g.sync()
     st = g.stats()
     g.set_timing(False)
     pe, ke, _ = g.energy()
     g.close()
+    if args.calibrate_copy:
+        import torch
+        src = torch.empty(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     ms_per_step = wall / args.steps * 1e3
-    pairs = int(st.pair_evals)
+    pairs = int(pairs_last)
     value = pairs / (ms_per_step * 1e-3)
     ns_day = (1e3 / ms_per_step) * s.dt * 0.0864
     flops = FLOPS_PER_PAIR[coul] * pairs
@@ -175,7 +244,10 @@ def main():
         },
         "ns_per_day": ns_day,
         "force_kernel_ms": st.force_ms,
-        "step_device_ms": st.step_ms,
+        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms),
+        "dt_fs": s.dt,
+        "step_device_ms_eager": st.step_ms,
+        "graph_captured_steps": bool(args.graph),
         "energy_kj_mol": {"potential_initial": pe_start, "potential_relaxed": pe_relaxed,
                           "kinetic_relaxed": ke0, "potential_end": pe, "kinetic_end": ke},
         "pre_relaxation": {"iterations": relax, "max_disp_A": args.relax_step,
@@ -192,6 +264,9 @@ def main():
             "flops_per_pair": FLOPS_PER_PAIR[coul],
         },
     }
+    if args.replicas:
+        out["batched_vs_plain"] = batched_vs_plain(args.replicas, args.replica_steps, args.seed,
+                                                   mdrl.Table.load())
     if not args.no_cpu:
         base, err = cpu_baseline_leg(s, fix0, pairs, args.cpu_seconds)
         out["cpu_baseline"] = base

@@ -210,8 +210,8 @@ int md_rl_set_graph(md_rl_ctx *c, int on);
  * 64; below 64, 64/apw lanes share a home atom and split its neighbours); tile_depth:
  * neighbour atoms staged in LDS per pass (128 or 256); offset_groups: workgroups per home
  * cell, splitting its 14 half-shell offsets (1, 2 or 4); persistent: a fixed grid draining
- * a largest-first task queue instead of one workgroup per task.  All schedules give
- * bit-identical int64 forces. */
+ * a largest-first task queue instead of one workgroup per task.  Default {64, 256, 1, 1}.
+ * All schedules give bit-identical int64 forces. */
 typedef struct md_rl_schedule {
     int atoms_per_wave;
     int tile_depth;

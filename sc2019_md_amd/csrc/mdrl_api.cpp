@@ -368,7 +368,7 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     if (!st) st = dalloc(&s.pair_ab_e, (size_t)nt * nt);
     if (!st) st = dalloc(&s.hk, nt);
     if (!st) st = dalloc(&s.mass, nt);
-    if (!st) st = dalloc(&s.counters, 8);
+    if (!st) st = dalloc(&s.counters, kCounterWords);
     if (!st) st = dalloc(&s.cell_red, 10 * (size_t)s.ncells);
     if (!st) st = dalloc(&s.atom_ke, n);
     if (!st) st = dalloc(&s.red, (size_t)kRedStride * nsys);
@@ -403,7 +403,7 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
         for (int k = 0; k < G.nc[0] * G.nc[1] * G.nc[2]; ++k) cell_sys[G.cell_off + k] = r;
         for (int k = 0; k < G.n; ++k) slot_sys[G.atom_off + k] = r;
     }
-    unsigned long long zeros[8] = {0};
+    unsigned long long zeros[kCounterWords] = {0};
 #define UP(dst, src, bytes)                                                              \
     if ((e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess) {          \
         md_rl_destroy(c);                                                                 \
@@ -868,11 +868,16 @@ extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[
 extern "C" int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *out)
 {
     if (!c || !out) { set_error("md_rl_get_stats: bad argument"); return MDRL_EINVAL; }
-    unsigned long long ctr[8];
+    unsigned long long ctr[kCounterWords];
     hipError_t e = hipMemcpy(ctr, c->s.counters, sizeof ctr, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
-    out->pair_evals = (int64_t)ctr[0];
-    out->candidates = (int64_t)ctr[1];
+    unsigned long long pairs = 0, cand = 0;
+    for (int k = 0; k < kStatSlots; ++k) {
+        pairs += ctr[8 + 2 * k];
+        cand += ctr[8 + 2 * k + 1];
+    }
+    out->pair_evals = (int64_t)pairs;
+    out->candidates = (int64_t)cand;
     out->flags = (int)ctr[2];
     out->max_cell_count = (int)ctr[3];
     out->force_ms = c->force_ms;

@@ -90,6 +90,15 @@ __device__ __forceinline__ void atomic_sub_wg(T *p, T v)
     __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Workgroup barrier that orders LDS only: global loads, stores and atomics issued before
+// it stay in flight (a full __syncthreads() would wait for every one of them).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Cell of one coordinate: (int)(x * inv_edge) clamped to [0, nc-1]; the clamp comes
 // first so a non-finite coordinate maps to a valid cell (NaN -> 0).
 __device__ __forceinline__ int cell_of(float x, float inv_edge, int nc)

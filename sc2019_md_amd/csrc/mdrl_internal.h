@@ -47,8 +47,8 @@ struct SysGeom {
 struct Schedule {
     int apw = 64;
     int tile = 256;
-    int groups = 4;
-    int persistent = 0;
+    int groups = 1;
+    int persistent = 1;
 };
 
 // Table window kept in LDS by the force kernel: entries [lo, lo + n).
@@ -56,6 +56,15 @@ struct TableWindow {
     int lo = 0, n = 0;
     int stride = 6;                     // floats per entry (6 with r^-3, 4 without)
 };
+
+// Pair and candidate counts are spread over kStatSlots counter pairs (workgroup b adds
+// into slot b mod kStatSlots), so thousands of workgroups do not serialise on one address.
+constexpr int kStatSlots = 64;
+// The persistent force kernel drains its task list through kQueues counters: queue c
+// serves the tasks c, c + kQueues, ... to the workgroups b with b mod kQueues = c.
+constexpr int kQueues = 8;
+constexpr int kQueueWord = 8 + 2 * kStatSlots;
+constexpr int kCounterWords = kQueueWord + kQueues;
 
 constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], -
 
@@ -92,7 +101,9 @@ struct DeviceSystem {
     float2 *pair_ab_e = nullptr;              // ntypes^2 (eps sig^12, eps sig^6)
     double *hk = nullptr;                     // ntypes: half-kick factor
     double *mass = nullptr;                   // ntypes
-    unsigned long long *counters = nullptr;   // [0] pairs [1] candidates [2] flags [3] max cell [4] work
+    unsigned long long *counters = nullptr;   // kCounterWords: [2] flags [3] max cell, then
+                                              // kStatSlots x {pairs, candidates}, then the
+                                              // kQueues work counters
     double *cell_red = nullptr;               // ncells x 10: PE + virial partials per home cell
     double *atom_ke = nullptr;                // n: kinetic energy per slot
     double *red = nullptr;                    // nsys x kRedStride

@@ -38,6 +38,13 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 constexpr int TI = 256, QCAP = 128, NWAVE = 8, NTHR = 512, NCHUNK = 4;
 
+// One neighbour tile: offset o, first atom jt of the tile within its cell, slot of that
+// atom, atoms in the tile and the periodic image shift of the cell.
+struct NbrTile {
+    int o, jt, base, mj;
+    float sx, sy, sz;
+};
+
 struct ForceArgs {
     const float4 *pos;
     const int *type;
@@ -73,6 +80,7 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
     __shared__ unsigned s_q[NWAVE][QCAP];
     __shared__ float2 s_ab[64];
     __shared__ int s_task;
+    __shared__ unsigned long long s_stat[NWAVE][2];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -100,10 +108,20 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
     for (int it = 0;; ++it) {
         int task;
         if (a.tasks) {
-            __syncthreads();
-            if (tid == 0) s_task = (int)atomicAdd(reinterpret_cast<unsigned *>(&a.counters[4]), 1u);
-            __syncthreads();
-            const int t = s_task;
+            // the first task of each workgroup is its own index; later ones come from its
+            // queue (only when there are more tasks than workgroups): queue c hands out the
+            // tasks c + kQueues * k, k = (workgroups of the queue) + its counter
+            int t = (int)blockIdx.x;
+            if (it > 0) {
+                if ((int)gridDim.x >= a.ntasks) break;
+                const int c = (int)blockIdx.x % kQueues;
+                const int nb = ((int)gridDim.x - c + kQueues - 1) / kQueues;
+                lds_barrier();
+                if (tid == 0)
+                    s_task = c + kQueues * (nb + (int)atomicAdd(reinterpret_cast<unsigned *>(&a.counters[kQueueWord + c]), 1u));
+                lds_barrier();
+                t = s_task;
+            }
             if (t >= a.ntasks) break;
             task = a.tasks[t];
         } else {
@@ -116,135 +134,172 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
         const int hl = h - G.cell_off;
         const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
         const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
+        const int obeg = a.gstart[grp], oend = a.gstart[grp + 1];
 
         for (int itile = 0; itile < hcount; itile += TI) {
             const int mi = min(TI, hcount - itile);
-            __syncthreads();
+            // The first neighbour tile at or after (offset o, tile jt) of this group that
+            // holds pairs: in the home cell only tiles reaching the diagonal.
+            auto find = [&](int o, int jt) {
+                NbrTile t{oend, 0, 0, 0, 0.f, 0.f, 0.f};
+                for (; o < oend; ++o, jt = 0) {
+                    int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
+                    t.sx = t.sy = t.sz = 0.f;
+                    if (rx < 0) { rx += nc0; t.sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; t.sx = G.box[0]; }
+                    if (ry < 0) { ry += nc1; t.sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; t.sy = G.box[1]; }
+                    if (rz < 0) { rz += nc2; t.sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; t.sz = G.box[2]; }
+                    const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
+                    const int cstart = a.cell_start[c], ccount = a.cell_start[c + 1] - cstart;
+                    if (o == 0 && jt < itile) jt = max(jt, (itile / TJ) * TJ);   // below the diagonal
+                    if (jt < ccount) {
+                        t.o = o; t.jt = jt; t.base = cstart + jt; t.mj = min(TJ, ccount - jt);
+                        return t;
+                    }
+                }
+                t.o = oend; t.jt = 0; t.base = 0; t.mj = 0;
+                t.sx = t.sy = t.sz = 0.f;
+                return t;
+            };
+            // stage the home tile and the first neighbour tile; the neighbour tile of the
+            // next pass is loaded into registers before a pass's pairs and written to LDS
+            // after it, so its latency hides behind the pass
+            NbrTile cur = find(obeg, 0);
             for (int k = tid; k < mi; k += NTHR) {
                 s_i[k] = a.pos[hstart + itile + k];
                 s_it[k] = a.type[hstart + itile + k];
             }
             for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
+            if (tid < cur.mj) {
+                float4 p = a.pos[cur.base + tid];
+                if (!PAR) { p.x = p.x + cur.sx; p.y = p.y + cur.sy; p.z = p.z + cur.sz; }
+                s_j[tid] = p;
+                s_jt[tid] = a.type[cur.base + tid];
+            }
+            for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
+            lds_barrier();
 
-            for (int o = a.gstart[grp]; o < a.gstart[grp + 1]; ++o) {
-                const bool self = (o == 0);
-                int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
-                float sx = 0.f, sy = 0.f, sz = 0.f;
-                if (rx < 0) { rx += nc0; sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; sx = G.box[0]; }
-                if (ry < 0) { ry += nc1; sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; sy = G.box[1]; }
-                if (rz < 0) { rz += nc2; sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; sz = G.box[2]; }
-                const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
-                const int cstart = a.cell_start[c], ccount = a.cell_start[c + 1] - cstart;
+            while (cur.o < oend) {
+                const NbrTile nxt = find(cur.o, cur.jt + TJ);
+                float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+                int tf = 0;
+                if (tid < nxt.mj) {
+                    pf = a.pos[nxt.base + tid];
+                    tf = a.type[nxt.base + tid];
+                }
+                const bool self = (cur.o == 0);
+                const int jtile = cur.jt, mj = cur.mj;
+                const float sx = cur.sx, sy = cur.sy, sz = cur.sz;
 
-                for (int jtile = 0; jtile < ccount; jtile += TJ) {
-                    if (self && jtile + TJ <= itile) continue;   // all pairs below the diagonal
-                    const int mj = min(TJ, ccount - jtile);
-                    __syncthreads();
-                    for (int k = tid; k < mj; k += NTHR) {
-                        float4 p = a.pos[cstart + jtile + k];
-                        if (!PAR) { p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; }
-                        s_j[k] = p;
-                        s_jt[k] = a.type[cstart + jtile + k];
+                auto eval = [&](unsigned e) {
+                    const int ia = (int)(e & 0xffffu), jb = (int)(e >> 16);
+                    const float4 pa = s_i[ia], pb = s_j[jb];
+                    float dx, dy, dz;
+                    const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
+                    const float2 ab = s_ab[s_it[ia] * a.ntypes + s_jt[jb]];
+                    const float qq = pa.w * pb.w;
+                    const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
+                                                                     a.off, a.nseg, a.win_lo, a.nwin);
+                    const float fx = F * dx, fy = F * dy, fz = F * dz;
+                    if (F32) {
+                        atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
+                        atomic_add_wg(&s_iacc_f[3 * ia + 1], fy);
+                        atomic_add_wg(&s_iacc_f[3 * ia + 2], fz);
+                        atomic_add_wg(&s_jacc_f[3 * jb + 0], -fx);
+                        atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
+                        atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
+                    } else {
+                        const long long kx = fix_of(fx, flag), ky = fix_of(fy, flag), kz = fix_of(fz, flag);
+                        atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
+                        atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
+                        atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
+                        atomic_sub_wg(&s_jacc[3 * jb + 0], (unsigned long long)kx);
+                        atomic_sub_wg(&s_jacc[3 * jb + 1], (unsigned long long)ky);
+                        atomic_sub_wg(&s_jacc[3 * jb + 2], (unsigned long long)kz);
                     }
-                    for (int k = tid; k < 3 * mj; k += NTHR) s_jacc[k] = 0ull;
-                    __syncthreads();
+                };
 
-                    auto eval = [&](unsigned e) {
-                        const int ia = (int)(e & 0xffffu), jb = (int)(e >> 16);
-                        const float4 pa = s_i[ia], pb = s_j[jb];
-                        float dx, dy, dz;
-                        const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
-                        const float2 ab = s_ab[s_it[ia] * a.ntypes + s_jt[jb]];
-                        const float qq = pa.w * pb.w;
-                        const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
-                                                                         a.off, a.nseg, a.win_lo, a.nwin);
-                        const float fx = F * dx, fy = F * dy, fz = F * dz;
-                        if (F32) {
-                            atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
-                            atomic_add_wg(&s_iacc_f[3 * ia + 1], fy);
-                            atomic_add_wg(&s_iacc_f[3 * ia + 2], fz);
-                            atomic_add_wg(&s_jacc_f[3 * jb + 0], -fx);
-                            atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
-                            atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
-                        } else {
-                            const long long kx = fix_of(fx, flag), ky = fix_of(fy, flag), kz = fix_of(fz, flag);
-                            atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
-                            atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
-                            atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
-                            atomic_sub_wg(&s_jacc[3 * jb + 0], (unsigned long long)kx);
-                            atomic_sub_wg(&s_jacc[3 * jb + 1], (unsigned long long)ky);
-                            atomic_sub_wg(&s_jacc[3 * jb + 2], (unsigned long long)kz);
-                        }
-                    };
-
-                    unsigned head = 0, tail = 0;
-                    const int bj = il % mj;
-                    const int nsteps = (mj + P - 1) / P;
-                    for (int cg0 = 0; cg0 < mi; cg0 += NCHUNK * APW) {
-                        if (self && itile + cg0 >= jtile + mj) break;   // rest is above the diagonal
-                        const int nch = min(NCHUNK, (mi - cg0 + APW - 1) / APW);
-                        // this lane's home atoms (far away past the tile: r2 = inf never
-                        // passes) and, in the home cell, the neighbour index each pair must
-                        // exceed so that a pair is taken once as (lower, higher)
-                        float4 pi[NCHUNK];
-                        int lim[NCHUNK];
+                unsigned head = 0, tail = 0;
+                const int bj = il % mj;
+                const int nsteps = (mj + P - 1) / P;
+                for (int cg0 = 0; cg0 < mi; cg0 += NCHUNK * APW) {
+                    if (self && itile + cg0 >= jtile + mj) break;   // rest is above the diagonal
+                    const int nch = min(NCHUNK, (mi - cg0 + APW - 1) / APW);
+                    // this lane's home atoms (far away past the tile: r2 = inf never
+                    // passes) and, in the home cell, the neighbour index each pair must
+                    // exceed so that a pair is taken once as (lower, higher)
+                    float4 pi[NCHUNK];
+                    int lim[NCHUNK];
+#pragma unroll
+                    for (int ch = 0; ch < NCHUNK; ++ch) {
+                        const int ia = cg0 + ch * APW + il;
+                        pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+                        lim[ch] = self ? itile + ia - jtile : -1;
+                    }
+                    for (int sft = wave; sft < nsteps; sft += NWAVE) {
+                        const int k = ph + sft * P;
+                        const bool kv = (P == 1) || (k < mj);
+                        int jb = bj + k;
+                        jb = (jb >= mj) ? jb - mj : jb;
+                        jb = kv ? jb : 0;
+                        const float4 pj = s_j[jb];
 #pragma unroll
                         for (int ch = 0; ch < NCHUNK; ++ch) {
-                            const int ia = cg0 + ch * APW + il;
-                            pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
-                            lim[ch] = self ? itile + ia - jtile : -1;
-                        }
-                        for (int sft = wave; sft < nsteps; sft += NWAVE) {
-                            const int k = ph + sft * P;
-                            const bool kv = (P == 1) || (k < mj);
-                            int jb = bj + k;
-                            jb = (jb >= mj) ? jb - mj : jb;
-                            jb = kv ? jb : 0;
-                            const float4 pj = s_j[jb];
-#pragma unroll
-                            for (int ch = 0; ch < NCHUNK; ++ch) {
-                                if (ch < nch) {
-                                    float dx, dy, dz;
-                                    const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
-                                    const int ia = cg0 + ch * APW + il;
-                                    const bool pass = kv && (r2 > 0.0f) && (r2 <= rc2) && (jb > lim[ch]);
-                                    const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
-                                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
-                                    tail += (unsigned)__popcll(m);
-                                    if (tail - head >= 64u) {
-                                        __builtin_amdgcn_wave_barrier();
-                                        eval(q[(head + lane) & (QCAP - 1)]);
-                                        __builtin_amdgcn_wave_barrier();
-                                        head += 64u;
-                                    }
+                            if (ch < nch) {
+                                float dx, dy, dz;
+                                const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
+                                const int ia = cg0 + ch * APW + il;
+                                const bool pass = kv && (r2 > 0.0f) && (r2 <= rc2) && (jb > lim[ch]);
+                                const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                                if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
+                                tail += (unsigned)__popcll(m);
+                                if (tail - head >= 64u) {
+                                    __builtin_amdgcn_wave_barrier();
+                                    eval(q[(head + lane) & (QCAP - 1)]);
+                                    __builtin_amdgcn_wave_barrier();
+                                    head += 64u;
                                 }
                             }
                         }
-                        ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
                     }
-                    npairs += tail;
-                    if (tail != head) {
-                        __builtin_amdgcn_wave_barrier();
-                        if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    __syncthreads();
-                    // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)
-                    if (F32) {
-                        for (int k = tid; k < 3 * mj; k += NTHR) {
-                            float v = s_jacc_f[k];
-                            if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)(cstart + jtile + k / 3) + k % 3], v);
+                    ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
+                }
+                npairs += tail;
+                if (tail != head) {
+                    __builtin_amdgcn_wave_barrier();
+                    if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
+                    __builtin_amdgcn_wave_barrier();
+                }
+                lds_barrier();
+                // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)
+                // and clear the accumulators for the next pass; the atomics stay in flight
+                const int nclr = 3 * max(mj, nxt.mj);
+                if (F32) {
+                    float *dst = a.f32 + 4 * (size_t)cur.base;
+                    for (int k = tid; k < nclr; k += NTHR) {
+                        if (k < 3 * mj) {
+                            const float v = s_jacc_f[k];
+                            if (v != 0.0f) atomic_add_agent(&dst[4 * (k / 3) + k % 3], v);
                         }
-                    } else {
-                        long long *dst = a.fix + 3 * (size_t)(cstart + jtile);
-                        for (int k = tid; k < 3 * mj; k += NTHR) {
-                            unsigned long long v = s_jacc[k];
+                        s_jacc_f[k] = 0.f;
+                    }
+                } else {
+                    long long *dst = a.fix + 3 * (size_t)cur.base;
+                    for (int k = tid; k < nclr; k += NTHR) {
+                        if (k < 3 * mj) {
+                            const unsigned long long v = s_jacc[k];
                             if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(dst + k), v);
                         }
+                        s_jacc[k] = 0ull;
                     }
                 }
+                if (tid < nxt.mj) {
+                    if (!PAR) { pf.x = pf.x + nxt.sx; pf.y = pf.y + nxt.sy; pf.z = pf.z + nxt.sz; }
+                    s_j[tid] = pf;
+                    s_jt[tid] = tf;
+                }
+                lds_barrier();
+                cur = nxt;
             }
-            __syncthreads();
             // flush reference-particle forces (D/Partial_Force_Acc.v:65-129)
             if (F32) {
                 for (int k = tid; k < 3 * mi; k += NTHR) {
@@ -258,12 +313,19 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
                     if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(dst + k), v);
                 }
             }
+            lds_barrier();
         }
     }
-    // per-wave statistics (npairs/ncand are wave-uniform)
+    // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
-        atomic_add_agent(&a.counters[0], npairs);
-        atomic_add_agent(&a.counters[1], ncand * 64ull);
+        s_stat[wave][0] = npairs;
+        s_stat[wave][1] = ncand * 64ull;
+    }
+    lds_barrier();
+    if (tid < 2) {
+        unsigned long long v = 0;
+        for (int w = 0; w < NWAVE; ++w) v += s_stat[w][tid];
+        atomic_add_agent(&a.counters[8 + 2 * (blockIdx.x % kStatSlots) + tid], v);
     }
     if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -371,7 +433,7 @@ __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const in
     run_add(count, c, s < n);
 }
 
-// Exclusive scan of the next-step counts into cell_start; clears counts and fill
+// pointers; records the largest cell; clears the pair counters and the work queues of
 // pointers; records the largest cell; clears the pair counters and the work counter of
 // the force evaluation that follows.  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *start, int *fill,
@@ -406,10 +468,8 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
     if (t == 1023) {
         start[ncells] = s_sum[1023];
         counters[3] = (unsigned long long)s_max[1023];
-        counters[0] = 0ull;
-        counters[1] = 0ull;
-        counters[4] = 0ull;
     }
+    if (t < 2 * kStatSlots + kQueues) counters[8 + t] = 0ull;
 }
 
 __global__ void k_scatter(int n, const int *cell, const int *start, int *fill, int *perm)
@@ -777,7 +837,7 @@ int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
 __global__ void k_zero_forces(int n, long long *fix, float *f32, unsigned long long *counters)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) { counters[0] = 0ull; counters[1] = 0ull; counters[4] = 0ull; }
+    if (s < 2 * kStatSlots + kQueues) counters[8 + s] = 0ull;
     if (s >= n) return;
     if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
     if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -786,7 +846,7 @@ __global__ void k_zero_forces(int n, long long *fix, float *f32, unsigned long l
 int launch_zero_forces(DeviceSystem &s, hipStream_t st)
 {
     const bool f32 = s.flags & MDRL_ACC_FP32;
-    hipLaunchKernelGGL(k_zero_forces, dim3(nblk(std::max(s.n, 1), 256)), dim3(256), 0, st, s.n,
+    hipLaunchKernelGGL(k_zero_forces, dim3(nblk(std::max(s.n, 2 * kStatSlots + kQueues), 256)), dim3(256), 0, st, s.n,
                        f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.counters);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;

@@ -67,6 +67,10 @@ class Schedule(ctypes.Structure):
         return "Schedule(apw=%d, tile=%d, groups=%d, persistent=%d)" % (
             self.atoms_per_wave, self.tile_depth, self.offset_groups, self.persistent)
 
+    def as_dict(self):
+        return {"atoms_per_wave": self.atoms_per_wave, "tile_depth": self.tile_depth,
+                "offset_groups": self.offset_groups, "persistent": self.persistent}
+
 
 class CModelCfg(ctypes.Structure):
     _fields_ = [
@@ -357,7 +361,7 @@ class System:
     def set_graph(self, on=True):
         _check(lib().md_rl_set_graph(self.handle, 1 if on else 0))
 
-    def set_schedule(self, apw=64, tile=256, groups=4, persistent=0):
+    def set_schedule(self, apw=64, tile=256, groups=1, persistent=1):
         sc = Schedule(apw, tile, groups, persistent)
         _check(lib().md_rl_set_schedule(self.handle, ctypes.byref(sc)))
 
