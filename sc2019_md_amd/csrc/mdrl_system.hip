@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "mdrl_device.h"
 
@@ -15,7 +16,7 @@ namespace mdrl {
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 // ---------------------------------------------------------------------------------
-// k_rl_halfshell: one workgroup of 8 waves per task = (home cell, group of half-shell
+// k_rl_halfshell: one workgroup of 12 waves per task = (home cell, group of half-shell
 // offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
 // first) from a counter.
 //
@@ -25,7 +26,7 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // 64/APW lanes share a home atom and split its neighbours).  At step s, lane l tests the
 // neighbour (l mod APW) + (l / APW) + s*64/APW (mod m) against all its home atoms, so within
 // a step both indices of the queued pairs are distinct and one LDS read feeds every chunk;
-// steps are strided over the 8 waves.  The r2 <= rc2 test of D/Filter_Logic.v:234 is a
+// steps are strided over the 12 waves.  The r2 <= rc2 test of D/Filter_Logic.v:234 is a
 // ballot; passing pairs are compacted into a per-wave LDS ring (the filter buffers and
 // arbiter of D/Filter_Bank.v:548-606) and, whenever 64 are queued, all 64 lanes evaluate
 // them densely: table force from an LDS window, then the reference particle gets +F and the
@@ -36,7 +37,9 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // (D/Partial_Force_Acc.v:65-129).
 // ---------------------------------------------------------------------------------
 
-constexpr int TI = 256, QCAP = 128, NWAVE = 8, NTHR = 512, NCHUNK = 4;
+// 12 waves per workgroup: two workgroups (24 waves) fill a CU's LDS and its VGPRs at
+// 6 waves per SIMD.
+constexpr int TI = 256, QCAP = 128, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
 
 // One neighbour tile: offset o, first atom jt of the tile within its cell, slot of that
 // atom, atoms in the tile and the periodic image shift of the cell.
@@ -66,7 +69,7 @@ struct ForceArgs {
 };
 
 template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ>
-__global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_rl_halfshell(ForceArgs a)
 {
     constexpr int STR = COUL ? 6 : 4;
     constexpr int P = 64 / APW;
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
     unsigned *q = s_q[wave];
     unsigned flag = 0;
     unsigned long long npairs = 0, ncand = 0;
-    const float rc2 = a.rc2;
+    const unsigned rc2_bits = __float_as_uint(a.rc2);
 
     for (int it = 0;; ++it) {
         int task;
@@ -229,38 +232,49 @@ __global__ __launch_bounds__(NTHR) void k_rl_halfshell(ForceArgs a)
                     // exceed so that a pair is taken once as (lower, higher)
                     float4 pi[NCHUNK];
                     int lim[NCHUNK];
+                    unsigned eb[NCHUNK];
 #pragma unroll
                     for (int ch = 0; ch < NCHUNK; ++ch) {
                         const int ia = cg0 + ch * APW + il;
                         pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
-                        lim[ch] = self ? itile + ia - jtile : -1;
+                        lim[ch] = itile + ia - jtile;
+                        eb[ch] = (unsigned)ia;
                     }
-                    for (int sft = wave; sft < nsteps; sft += NWAVE) {
-                        const int k = ph + sft * P;
-                        const bool kv = (P == 1) || (k < mj);
-                        int jb = bj + k;
-                        jb = (jb >= mj) ? jb - mj : jb;
-                        jb = kv ? jb : 0;
-                        const float4 pj = s_j[jb];
+                    // SELF: the home cell, where a pair needs jb > lim (compiled twice so the
+                    // other 13 offsets skip that test)
+                    auto run_steps = [&](auto self_c) {
+                        constexpr bool SELF = decltype(self_c)::value;
+                        for (int sft = wave; sft < nsteps; sft += NWAVE) {
+                            const int k = ph + sft * P;
+                            const bool kv = (P == 1) || (k < mj);
+                            int jb = bj + k;
+                            jb = (jb >= mj) ? jb - mj : jb;
+                            jb = kv ? jb : 0;
+                            const float4 pj = s_j[jb];
+                            const unsigned jsh = (unsigned)jb << 16;
 #pragma unroll
-                        for (int ch = 0; ch < NCHUNK; ++ch) {
-                            if (ch < nch) {
-                                float dx, dy, dz;
-                                const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
-                                const int ia = cg0 + ch * APW + il;
-                                const bool pass = kv && (r2 > 0.0f) && (r2 <= rc2) && (jb > lim[ch]);
-                                const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
-                                if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = (unsigned)ia | ((unsigned)jb << 16);
-                                tail += (unsigned)__popcll(m);
-                                if (tail - head >= 64u) {
-                                    __builtin_amdgcn_wave_barrier();
-                                    eval(q[(head + lane) & (QCAP - 1)]);
-                                    __builtin_amdgcn_wave_barrier();
-                                    head += 64u;
+                            for (int ch = 0; ch < NCHUNK; ++ch) {
+                                if (ch < nch) {
+                                    float dx, dy, dz;
+                                    const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
+                                    // 0 < r2 <= rc2 as one unsigned compare of the bit patterns (r2 >= +0)
+                                    const bool pass = kv && (__float_as_uint(r2) - 1u < rc2_bits) &&
+                                                      (!SELF || jb > lim[ch]);
+                                    const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = eb[ch] | jsh;
+                                    tail += (unsigned)__popcll(m);
+                                    if (tail - head >= 64u) {
+                                        __builtin_amdgcn_wave_barrier();
+                                        eval(q[(head + lane) & (QCAP - 1)]);
+                                        __builtin_amdgcn_wave_barrier();
+                                        head += 64u;
+                                    }
                                 }
                             }
                         }
-                    }
+                    };
+                    if (self) run_steps(std::true_type{});
+                    else run_steps(std::false_type{});
                     ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
                 }
                 npairs += tail;
