@@ -45,6 +45,23 @@ __device__ __forceinline__ long long fix_of(float f, unsigned &flag)
     return (long long)rint(x);
 }
 
+// fix_of of the three components of a pair force, with one range test for the fast path
+// (|f| summed, so a NaN or an infinity takes the general path).
+__device__ __forceinline__ void fix3(float fx, float fy, float fz, long long &kx, long long &ky,
+                                     long long &kz, unsigned &flag)
+{
+    const float m = (fabsf(fx) + fabsf(fy)) + fabsf(fz);
+    if (m < 524288.0f) {
+        kx = __double_as_longlong(fma((double)fx, 4294967296.0, 6755399441055744.0)) - 0x4338000000000000LL;
+        ky = __double_as_longlong(fma((double)fy, 4294967296.0, 6755399441055744.0)) - 0x4338000000000000LL;
+        kz = __double_as_longlong(fma((double)fz, 4294967296.0, 6755399441055744.0)) - 0x4338000000000000LL;
+    } else {
+        kx = fix_of(fx, flag);
+        ky = fix_of(fy, flag);
+        kz = fix_of(fz, flag);
+    }
+}
+
 __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -143,15 +160,16 @@ __device__ __forceinline__ float pair_geom(float4 pi, float4 pj, float sx, float
 // Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3).  Entries [win_lo, win_lo + nwin)
 // -- the bins reachable below rc2 down to a floor seven octaves lower -- are read from
 // LDS; the rare r2 outside (overlapping atoms, or the reference's segment-0 fallback)
-// re-reads its entry from the global copy.
+// re-reads its entry from the global copy.  Inside the window the entry is
+// (bits(r2) >> 15) - off*256 - win_lo: exponent and mantissa[22:15] of r2 in one shift,
+// with no clamp (the window lies inside segments [0, nseg)).
 template <bool COUL, int STR>
 __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const float *__restrict__ g_tab,
                                              int off, int nseg, int win_lo, int nwin, float c[6])
 {
-    const int seg = lut_seg(r2, off, nseg);
-    const int idx = seg * 256 + lut_bin(r2);
-    const bool inw = (unsigned)(idx - win_lo) < (unsigned)nwin;
-    const int li = inw ? idx - win_lo : 0;
+    const unsigned wi = (__float_as_uint(r2) >> 15) - (unsigned)(off * 256 + win_lo);
+    const bool inw = wi < (unsigned)nwin;
+    const int li = inw ? (int)wi : 0;
     if (STR == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(s_tab + li * 4);
         c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
@@ -162,9 +180,13 @@ __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const
         c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y; c[4] = w.x; c[5] = w.y;
     }
     if (!inw) {
+        const int idx = lut_seg(r2, off, nseg) * 256 + lut_bin(r2);
         const float *q = g_tab + (size_t)idx * 6;
         c[0] = q[0]; c[1] = q[1]; c[2] = q[2]; c[3] = q[3];
         if (COUL) { c[4] = q[4]; c[5] = q[5]; }
+        // complete these loads here, so that the common path does not wait on the
+        // caller's other global traffic (prefetches, flush atomics) in flight
+        __builtin_amdgcn_s_waitcnt(0);
     }
 }
 

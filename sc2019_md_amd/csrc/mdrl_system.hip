@@ -40,6 +40,7 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // 12 waves per workgroup: two workgroups (24 waves) fill a CU's LDS and its VGPRs at
 // 6 waves per SIMD.
 constexpr int TI = 256, QCAP = 128, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
+static_assert(TI <= 256, "queue entries hold 8-bit home indices");
 
 // One neighbour tile: offset o, first atom jt of the tile within its cell, slot of that
 // atom, atoms in the tile and the periodic image shift of the cell.
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
 {
     constexpr int STR = COUL ? 6 : 4;
     constexpr int P = 64 / APW;
+    static_assert(TJ <= 256 && TJ <= NTHR, "queue entries hold 8-bit neighbour indices");
     extern __shared__ float s_tab[];
     __shared__ float4 s_i[TI];
     __shared__ float4 s_j[TJ];
@@ -193,12 +195,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const int jtile = cur.jt, mj = cur.mj;
                 const float sx = cur.sx, sy = cur.sy, sz = cur.sz;
 
+                // queue entry: home index | neighbour index << 8 | type pair << 16
                 auto eval = [&](unsigned e) {
-                    const int ia = (int)(e & 0xffffu), jb = (int)(e >> 16);
+                    const int ia = (int)(e & 0xffu), jb = (int)((e >> 8) & 0xffu);
                     const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
                     const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
-                    const float2 ab = s_ab[s_it[ia] * a.ntypes + s_jt[jb]];
+                    const float2 ab = s_ab[e >> 16];
                     const float qq = pa.w * pb.w;
                     const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
                                                                      a.off, a.nseg, a.win_lo, a.nwin);
@@ -211,7 +214,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
                         atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
                     } else {
-                        const long long kx = fix_of(fx, flag), ky = fix_of(fy, flag), kz = fix_of(fz, flag);
+                        long long kx, ky, kz;
+                        fix3(fx, fy, fz, kx, ky, kz, flag);
                         atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
                         atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
                         atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
@@ -231,17 +235,16 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     // passes) and, in the home cell, the neighbour index each pair must
                     // exceed so that a pair is taken once as (lower, higher)
                     float4 pi[NCHUNK];
-                    int lim[NCHUNK];
                     unsigned eb[NCHUNK];
 #pragma unroll
                     for (int ch = 0; ch < NCHUNK; ++ch) {
                         const int ia = cg0 + ch * APW + il;
                         pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
-                        lim[ch] = itile + ia - jtile;
-                        eb[ch] = (unsigned)ia;
+                        eb[ch] = (unsigned)ia | ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << 16);
                     }
-                    // SELF: the home cell, where a pair needs jb > lim (compiled twice so the
-                    // other 13 offsets skip that test)
+                    // SELF: the home cell, where a pair needs jb > itile + ia - jtile
+                    // (compiled twice so the other 13 offsets skip that test)
+                    const int dlim = itile - jtile;
                     auto run_steps = [&](auto self_c) {
                         constexpr bool SELF = decltype(self_c)::value;
                         for (int sft = wave; sft < nsteps; sft += NWAVE) {
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                             jb = (jb >= mj) ? jb - mj : jb;
                             jb = kv ? jb : 0;
                             const float4 pj = s_j[jb];
-                            const unsigned jsh = (unsigned)jb << 16;
+                            const unsigned jsh = ((unsigned)jb << 8) | ((unsigned)s_jt[jb] << 16);
 #pragma unroll
                             for (int ch = 0; ch < NCHUNK; ++ch) {
                                 if (ch < nch) {
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                                     const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
                                     // 0 < r2 <= rc2 as one unsigned compare of the bit patterns (r2 >= +0)
                                     const bool pass = kv && (__float_as_uint(r2) - 1u < rc2_bits) &&
-                                                      (!SELF || jb > lim[ch]);
+                                                      (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
                                     const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
                                     if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = eb[ch] | jsh;
                                     tail += (unsigned)__popcll(m);
