@@ -203,11 +203,12 @@ def main():
     pe, ke, _ = g.energy()
     g.close()
     if args.calibrate_copy:
+        # a plain streaming kernel of known traffic (reads and writes calibrate_copy bytes)
+        # for scaling the HBM byte counters
         import torch
         src = torch.empty(args.calibrate_copy, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        mdrl.stream_copy(dst.data_ptr(), src.data_ptr(), args.calibrate_copy)
 
     ms_per_step = wall / args.steps * 1e3
     pairs = int(pairs_last)
@@ -215,9 +216,11 @@ def main():
     ns_day = (1e3 / ms_per_step) * s.dt * 0.0864
     flops = FLOPS_PER_PAIR[coul] * pairs
     achieved = flops / (st.force_ms * 1e-3) / 1e12
+    # HBM bytes per force launch from the committed counter passes (FETCH_SIZE and
+    # WRITE_SIZE, scaled by a copy kernel of known traffic); null when not measured
     traffic = None
     tf = os.path.join(ROOT, "profiles", "force_traffic_workload%d.json" % args.workload)
-    if os.path.exists(tf):
+    if os.path.exists(tf) and not (args.fp32_atomics or args.analytic):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
 
     out = {
@@ -262,6 +265,9 @@ def main():
             "traffic": traffic,
             "kernel": "k_rl_halfshell",
             "flops_per_pair": FLOPS_PER_PAIR[coul],
+            "hbm": None if traffic is None else {
+                "achieved": traffic / (st.force_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": traffic / (st.force_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         },
     }
     if args.replicas:

@@ -265,6 +265,12 @@ typedef struct md_rl_system {
 int md_rl_make_system(md_rl_system *s, int which, int index, uint64_t seed);
 void md_rl_free_system(md_rl_system *s);
 
+/* ---- diagnostics --------------------------------------------------------------------- */
+/* No counterpart in the reference.  Copies bytes (a multiple of 16) from device buffer
+ * src to device buffer dst with one plain streaming kernel and waits for it: a kernel of
+ * exactly known traffic (bytes read, bytes written) for scaling hardware byte counters. */
+int md_rl_stream_copy(void *dst, const void *src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

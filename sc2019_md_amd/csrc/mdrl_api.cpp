@@ -126,6 +126,16 @@ extern "C" int md_rl_eval_pairs_ex(const md_rl_table *t, const float *ref_xyz, c
     return eval_pairs_common(t, ref_xyz, nbr_xyz, p_a, p_b, p_qq, force_s, force_xyz, npairs, rc2, flags);
 }
 
+extern "C" int md_rl_stream_copy(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src || bytes % 16) { set_error("md_rl_stream_copy: bad argument"); return MDRL_EINVAL; }
+    int rc = check_device();
+    if (rc) return rc;
+    if ((rc = launch_stream_copy(dst, src, bytes, nullptr))) return rc;
+    MDRL_HIP(hipDeviceSynchronize());
+    return MDRL_OK;
+}
+
 extern "C" int md_rl_eval_pairs(const md_rl_table *t, const int32_t *ref_id, const int32_t *nbr_id,
                                 const float *ref_x, const float *ref_y, const float *ref_z,
                                 const float *nbr_x, const float *nbr_y, const float *nbr_z,

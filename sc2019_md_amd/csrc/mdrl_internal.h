@@ -130,6 +130,8 @@ int launch_eval_pairs(const float *d_tab, int exp_offset, int nseg, const float 
                       const float *nbr, const float *pa, const float *pb, const float *pqq,
                       float *fs, float *fxyz, int64_t n, float rc2, int flags, hipStream_t st);
 
+int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+
 struct CModelDev {
     int n;
     float *x, *v, *f, *e, *ke;   // x,v,f: 3 x n SoA; e, ke: n

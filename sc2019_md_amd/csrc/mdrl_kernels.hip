@@ -69,6 +69,24 @@ int launch_eval_pairs(const float *d_tab, int exp_offset, int nseg, const float 
     return MDRL_OK;
 }
 
+// Plain streaming copy (16 B per lane per iteration), for calibrating byte counters.
+__global__ void k_stream_copy(float4 *__restrict__ dst, const float4 *__restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return MDRL_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, static_cast<float4 *>(dst),
+                       static_cast<const float4 *>(src), n);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
 // ---------------------------------------------------------------------------------
 // The C model on the GPU: R/C_Model_ForceEval/MD_Evaluation_Model.cpp:57-203 with the
 // same promotions (double pow, double +-0.5 in the PBC, double 0.5*MASS), one thread

@@ -456,6 +456,12 @@ class Batch(System):
 
 
 # ---------------------------------------------------------------------------------- C model
+def stream_copy(dst_ptr, src_ptr, nbytes):
+    """md_rl_stream_copy on raw device addresses (a kernel of known traffic)."""
+    _check(lib().md_rl_stream_copy(ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr),
+                                   ctypes.c_size_t(nbytes)))
+
+
 def cmodel_default():
     cfg = CModelCfg()
     lib().md_rl_cmodel_default(ctypes.byref(cfg))
