@@ -13,6 +13,7 @@ sample of the same system in the same run) together with the force error against
 direct evaluation.
 """
 import argparse
+import copy
 import json
 import math
 import os
@@ -144,6 +145,70 @@ def batched_vs_plain(nrep, steps, seed, table):
     }
 
 
+def precision_leg(s, relax, relax_step, nve_steps, dt_list):
+    """The precision configuration: int64 forces bit-identical over 10 repeat
+    evaluations, FP32 atomics vs int64 and both vs the FP64 direct oracle, FP64-reduced
+    total energy along NVE runs (one per time step in dt_list)."""
+    import hashlib
+    from oracle import oracle as O
+
+    g = mdrl.System.from_synthetic(s)
+    if relax:
+        g.minimize(relax, relax_step, 1e-4)
+    digests = set()
+    for _ in range(10):
+        g.force_eval()
+        digests.add(hashlib.sha256(g.forces_fixed().tobytes()).hexdigest())
+    fix = g.forces_fixed()
+    relaxed = g.particles()
+    g.close()
+    h = mdrl.System.from_synthetic(s, flags=mdrl.ACC_FP32)
+    h.set_particles(relaxed)
+    f32a = h.forces_fixed()
+    h.force_eval()
+    f32b = h.forces_fixed()
+    h.close()
+    fd, _, _, _ = O.fp64_direct(relaxed[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                charge=s.charge, ke=s.coulomb_k)
+    rel = lambda a: float(np.linalg.norm(a.astype(np.float64) / 2 ** 32 * 0.1 - fd) / np.linalg.norm(fd))
+    out = {
+        "repeat_force_evals": 10,
+        "int64_bit_identical": len(digests) == 1,
+        "fp32_atomics_runs_bit_identical": bool(np.array_equal(f32a, f32b)),
+        "fp32_vs_int64_rel_l2": float(np.linalg.norm((f32a - fix).astype(np.float64)) /
+                                      np.linalg.norm(fix.astype(np.float64))),
+        "int64_vs_fp64_rel_l2": rel(fix),
+        "fp32_vs_fp64_rel_l2": rel(f32a),
+        "nve": [],
+    }
+    for dt in dt_list:
+        sd = copy.copy(s)
+        sd.dt = float(dt)
+        g = mdrl.System.from_synthetic(sd)
+        g.set_particles(relaxed)
+        g.set_graph(True)
+        pe0, ke0, _ = g.energy()
+        trace = [(0, pe0 + ke0)]
+        every = max(1, nve_steps // 10)
+        t0 = time.perf_counter()
+        done = 0
+        while done < nve_steps:
+            k = min(every, nve_steps - done)
+            g.step_async(k)
+            done += k
+            pe, ke, _ = g.energy()
+            trace.append((done, pe + ke))
+        wall = time.perf_counter() - t0
+        g.close()
+        e0 = trace[0][1]
+        out["nve"].append({"dt_fs": dt, "steps": nve_steps, "E0_kj_mol": e0,
+                           "E_end_kj_mol": trace[-1][1],
+                           "rel_drift": (trace[-1][1] - e0) / abs(e0),
+                           "max_rel_dev": max(abs(e - e0) for _, e in trace) / abs(e0),
+                           "trace": trace, "wall_s": wall})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +227,8 @@ def main():
     ap.add_argument("--replica-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1, help="capture the timed steps in hipGraphs")
     ap.add_argument("--dt", type=float, default=None, help="time step in fs (default: the workload's)")
+    ap.add_argument("--nve-steps", type=int, default=10000,
+                    help="NVE steps of the precision leg (workload 3)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one device copy of this many bytes (traffic counter calibration)")
     args = ap.parse_args()
@@ -270,9 +337,12 @@ def main():
                 "frac": traffic / (st.force_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         },
     }
-    if args.replicas:
+    if args.replicas and args.workload == 2:
         out["batched_vs_plain"] = batched_vs_plain(args.replicas, args.replica_steps, args.seed,
                                                    mdrl.Table.load())
+    if args.workload == 3:
+        out["precision"] = precision_leg(s, relax, args.relax_step, args.nve_steps,
+                                         sorted({s.dt, 0.5}))
     if not args.no_cpu:
         base, err = cpu_baseline_leg(s, fix0, pairs, args.cpu_seconds)
         out["cpu_baseline"] = base

@@ -12,6 +12,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* tables                                                                     */
@@ -462,14 +465,26 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
     cell_list cl;
     build_cells(n, s->pos4, s->box, s->ncell, &cl);
     memset(fix_out, 0, sizeof(int64_t) * 3 * (size_t)n);
-    uint64_t *acc = (uint64_t *)fix_out;   /* two's-complement wrap-around adds */
     const int par = s->mode & OR_MODE_PARITY;
     int64_t npairs = 0;
     const int *nc = s->ncell;
-    for (int hx = 0; hx < nc[0]; ++hx)
-    for (int hy = 0; hy < nc[1]; ++hy)
-    for (int hz = 0; hz < nc[2]; ++hz) {
-        int h = (hx * nc[1] + hy) * nc[2] + hz;
+    /* Home cells run in parallel, each thread into its own int64 accumulators; integer
+     * sums are order-free, so the result does not depend on the thread count. */
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    uint64_t *priv = NULL;
+    if (nthr > 1) priv = (uint64_t *)calloc((size_t)nthr * 3 * (size_t)n, sizeof(uint64_t));
+    if (!priv) nthr = 1;
+    #pragma omp parallel for schedule(dynamic, 1) reduction(+:npairs) num_threads(nthr) if(nthr > 1)
+    for (int h = 0; h < cl.ncells; ++h) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        uint64_t *acc = priv ? priv + (size_t)tid * 3 * (size_t)n : (uint64_t *)fix_out;
+        const int hx = h / (nc[1] * nc[2]), hy = (h / nc[2]) % nc[1], hz = h % nc[2];
         for (int o = 0; o < 14; ++o) {
             int raw[3] = {hx + HS[o][0], hy + HS[o][1], hz + HS[o][2]};
             float S[3];
@@ -511,6 +526,16 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
                 }
             }
         }
+    }
+    if (priv) {
+        uint64_t *out = (uint64_t *)fix_out;
+        #pragma omp parallel for schedule(static)
+        for (long long k = 0; k < 3LL * n; ++k) {
+            uint64_t v = 0;
+            for (int t = 0; t < nthr; ++t) v += priv[(size_t)t * 3 * (size_t)n + (size_t)k];
+            out[k] = v;
+        }
+        free(priv);
     }
     free_cells(&cl);
     return npairs;
