@@ -157,6 +157,23 @@ int md_rl_create(md_rl_ctx **out, const md_rl_params *p);
 int md_rl_set_particles(md_rl_ctx *c, const float *particle, int n);
 /* Type index (0..ntypes-1) and charge (e) per particle; charge may be NULL. */
 int md_rl_set_types(md_rl_ctx *c, const int32_t *type, const float *charge, int n);
+/* Bonded terms of system sys (0 for a single system), replacing any set before; they
+ * replace the streams of Bonded/Bonded_Top.v:202-380.  atoms[nterms][4] are the system's
+ * input indices with -1 padding: a bond (i, j, -1, -1), an angle (i, j, k, -1) with
+ * vertex j, or a torsion (i, j, k, l).  params[nterms][4], positions in A:
+ *   bond    (r0, k [kJ/mol/A^2]):            U = k (r - r0)^2           (Bonded/calc_bond_v2.v:83-110)
+ *   angle   (theta0 [rad], k [kJ/mol/rad^2]): U = k (theta - theta0)^2  (Bonded/calc_angle_v2.v:113-147)
+ *   torsion (k [kJ/mol], n in 1..6, phase [rad], -): U = k (1 + cos(n phi + phase))
+ *                                                                     (Bonded/calcTorsion_v2.v:70-274)
+ * FP32 forces join the range-limited forces in the same fixed-point buffer (and every
+ * md_rl_force_eval / step / minimize); their FP64 energy joins the potential energy. */
+int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t *atoms, const float *params);
+/* Nonbonded exclusions of system sys (no counterpart in the reference, whose golden
+ * models exclude by distance, M/FullScale_Simulation_LJArgon.m:801): bit d of mask[i]
+ * removes the pair (i, i + d - 16) (system input indices) from the range-limited forces,
+ * energies and pair counts.  Windows must be symmetric with bit 16 clear; NULL removes
+ * them.  Needs the default int64 table path. */
+int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask);
 /* Forces at the current positions (cols 6-8 after md_rl_get_particles). */
 int md_rl_force_eval(md_rl_ctx *c);
 /* nsteps velocity-Verlet (or kick-drift) steps: force, Motion_Update, rebin. */
@@ -259,9 +276,18 @@ typedef struct md_rl_system {
     int32_t *type;     /* n */
     float *charge;     /* n */
     float eps[8], sigma[8], mass[8];
+    /* bonded topology (workload 5; 0 / NULL otherwise), in md_rl_set_bonded's format */
+    int nterms;
+    int32_t *term_atoms;   /* nterms x 4 */
+    float *term_params;    /* nterms x 4 */
+    uint32_t *excl;        /* n exclusion windows, or NULL */
 } md_rl_system;
 /* which: 1 = 512-atom argon; 2 = 92,224-atom LJ+Coulomb; 3 = 1,048,576-atom;
- * 4 = one replica of the batched set (index = replica, 0..511). */
+ * 4 = one replica of the batched set (index = replica, 0..511);
+ * 5 = one replica of the bonded set (index = replica, 0..127): 64 linear chains of 48
+ *     united-atom beads at 0.033 beads/A^3 (45.33 A box, rc 10 A, dt 2 fs), bonds
+ *     r0 1.53 A, angles 111 deg, torsions n = 3, charges +-0.25 e alternating, LJ and
+ *     Coulomb excluded within 3 bonds. */
 int md_rl_make_system(md_rl_system *s, int which, int index, uint64_t seed);
 void md_rl_free_system(md_rl_system *s);
 

@@ -459,6 +459,14 @@ static inline float pair_scalar(const or_rl_system *s, float r2, int ti, int tj,
     return F;
 }
 
+/* Exclusion window: bit d of m[i] excludes the pair (i, i + d - 16); windows are
+ * symmetric, so testing either atom's window gives the same answer. */
+static inline int or_excluded(const uint32_t *m, int i, int j)
+{
+    const int d = j - i + 16;
+    return (unsigned)d < 32u && ((m[i] >> d) & 1u);
+}
+
 int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
 {
     const int n = s->n;
@@ -515,7 +523,7 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
                         r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     }
                     if (!(r2 > 0.0f && r2 <= s->rc2)) continue;
-                    if (s->excl_mask && s->excl_mask[i] && s->excl_mask[j]) { /* reserved */ }
+                    if (s->excl && or_excluded(s->excl, i, j)) continue;
                     ++npairs;
                     float qq = pi[3] * pj[3];
                     float F = pair_scalar(s, r2, s->type[i], s->type[j], qq);
@@ -575,6 +583,7 @@ int64_t or_table_fp64(const or_rl_system *s, double *f_out)
                     double dz = (double)pi[2] - (double)pj[2] - S[2];
                     double r2 = dx * dx + dy * dy + dz * dz;
                     if (!(r2 > 0.0 && r2 <= (double)s->rc2)) continue;
+                    if (s->excl && or_excluded(s->excl, i, j)) continue;
                     ++npairs;
                     int idx = or_lut_index((float)r2, s->exp_offset, s->nseg);
                     const float *cf = s->tab + 6 * (size_t)idx;
@@ -618,7 +627,8 @@ static inline void lj_coul_fp64(double r2, double eps, double sig, double qq, do
 int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *q,
                        int ntypes, const double *eps_ij, const double *sig_ij,
                        double ke, int coulomb, const double box[3], double rc,
-                       int use_cells, double *f_out, double *energy, double *virial)
+                       int use_cells, double *f_out, double *energy, double *virial,
+                       const uint32_t *excl)
 {
     const double rc2 = rc * rc;
     int64_t npairs = 0;
@@ -635,6 +645,7 @@ int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *
                 for (int ay = -im[1]; ay <= im[1]; ++ay)
                 for (int az = -im[2]; az <= im[2]; ++az) {
                     if (i == j && ax == 0 && ay == 0 && az == 0) continue;
+                    if (excl && or_excluded(excl, i, j)) continue;
                     double dx = xyz[3 * i] - xyz[3 * j] - ax * box[0];
                     double dy = xyz[3 * i + 1] - xyz[3 * j + 1] - ay * box[1];
                     double dz = xyz[3 * i + 2] - xyz[3 * j + 2] - az * box[2];
@@ -706,6 +717,7 @@ int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *
                     double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
                     double r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 > rc2 || r2 <= 0.0) continue;
+                    if (excl && or_excluded(excl, i, j)) continue;
                     int tt = type[i] * ntypes + type[j];
                     double F, E;
                     lj_coul_fp64(r2, eps_ij[tt], sig_ij[tt], q ? q[i] * q[j] : 0.0, ke,
@@ -788,4 +800,301 @@ void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,
         ke += 0.5 * mass[type[i]] * vv * 1.0e4;
     }
     if (ke_out) *ke_out = ke;
+}
+
+/* ---- bonded terms ---------------------------------------------------------------- */
+
+/* Minimum image of one FP32 component (positions are wrapped into [0, L)). */
+static inline float img_f(float d, float L)
+{
+    const float h = 0.5f * L;
+    if (d > h) return d - L;
+    if (d < -h) return d + L;
+    return d;
+}
+static inline float dot3f(const float *a, const float *b)
+{
+    float r = a[0] * b[0] + a[1] * b[1];
+    return r + a[2] * b[2];
+}
+static inline void cross3f(const float *a, const float *b, float *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* atan2 for y >= 0: reduce to a = min/max in [0, 1]; above tan(pi/12) use
+ * atan a = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3)), leaving |t| <= tan(pi/12) = 0.268 for
+ * the Taylor series t - t^3/3 + ... - t^11/11 (next term < 3e-9). */
+float or_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    const float a = mn / mx;
+    float t = a, off = 0.0f;
+    if (a > 0.26794919f) {
+        const float num = a * 1.7320508f - 1.0f;
+        const float den = a + 1.7320508f;
+        t = num / den;
+        off = 0.52359878f;
+    }
+    const float t2 = t * t;
+    float p = -0.09090909f;          /* -1/11 */
+    p = p * t2 + 0.11111111f;        /*  1/9  */
+    p = p * t2 - 0.14285715f;        /* -1/7  */
+    p = p * t2 + 0.2f;               /*  1/5  */
+    p = p * t2 - 0.33333334f;        /* -1/3  */
+    p = p * t2 + 1.0f;
+    float r = p * t;
+    r = off + r;
+    if (ay > ax) r = 1.5707964f - r;
+    if (x < 0.0f) r = 3.1415927f - r;
+    return y < 0.0f ? -r : r;
+}
+
+static inline void to_fix(const float *f, int64_t *k)
+{
+    k[0] = or_fix(f[0]); k[1] = or_fix(f[1]); k[2] = or_fix(f[2]);
+}
+static inline void add_k(int64_t *fix, int a, const int64_t *k)
+{
+    uint64_t *u = (uint64_t *)fix;
+    u[3 * (size_t)a + 0] += (uint64_t)k[0];
+    u[3 * (size_t)a + 1] += (uint64_t)k[1];
+    u[3 * (size_t)a + 2] += (uint64_t)k[2];
+}
+static inline void add_fix(int64_t *fix, int a, const float *f)
+{
+    int64_t k[3];
+    to_fix(f, k);
+    add_k(fix, a, k);
+}
+
+void or_bonded_fix(int n, const float *pos4, const float box[3], int nterms,
+                   const int32_t *atoms, const float *params, int64_t *fix_out,
+                   double *term_energy)
+{
+    (void)n;
+    for (int t = 0; t < nterms; ++t) {
+        const int32_t *at = atoms + 4 * (size_t)t;
+        const float *pp = params + 4 * (size_t)t;
+        const float *x1 = pos4 + 4 * (size_t)at[0], *x2 = pos4 + 4 * (size_t)at[1];
+        double e = 0.0;
+        if (at[2] < 0) {
+            /* bond, Bonded/calc_bond_v2.v:83-110: r12 = x1 - x2, dpot/dr = 2 k (r - r0),
+             * f1 = (-dpot/dr / r) r12, f2 = -f1 */
+            float r12[3];
+            for (int d = 0; d < 3; ++d) r12[d] = img_f(x1[d] - x2[d], box[d]);
+            const float r = sqrtf(dot3f(r12, r12));
+            const float dr = r - pp[0];
+            float dp = 2.0f * pp[1];
+            dp = dp * dr;
+            const float c = -dp / r;
+            float f1[3], f2[3];
+            for (int d = 0; d < 3; ++d) { f1[d] = c * r12[d]; f2[d] = -f1[d]; }
+            add_fix(fix_out, at[0], f1);
+            add_fix(fix_out, at[1], f2);
+            const double ddr = (double)r - (double)pp[0];
+            e = (double)pp[1] / 10.0 * ddr * ddr;
+        } else if (at[3] < 0) {
+            /* angle, Bonded/calc_angle_v2.v:113-147 (no Urey-Bradley term):
+             * theta = atan2(|r12 x r32|, r12 . r32), dpot/dtheta = 2 k (theta - theta0),
+             * dtheta1 = (r^12 cos - r^32) / (sin d12), dtheta3 = (r^32 cos - r^12) / (sin d32),
+             * f1 = -dpot dtheta1, f3 = -dpot dtheta3, f2 = -f1 - f3 */
+            const float *x3 = pos4 + 4 * (size_t)at[2];
+            float r12[3], r32[3], cr[3];
+            for (int d = 0; d < 3; ++d) {
+                r12[d] = img_f(x1[d] - x2[d], box[d]);
+                r32[d] = img_f(x3[d] - x2[d], box[d]);
+            }
+            const float d12 = sqrtf(dot3f(r12, r12)), d32 = sqrtf(dot3f(r32, r32));
+            cross3f(r12, r32, cr);
+            const float y = sqrtf(dot3f(cr, cr)), xx = dot3f(r12, r32);
+            const float th = or_atan2f(y, xx);
+            const float dd = d12 * d32;
+            const float cs = xx / dd, sn = y / dd;
+            {
+                double dth = (double)th - (double)pp[0];
+                e = (double)pp[1] / 10.0 * dth * dth;
+            }
+            if (sn > 1e-6f) {
+                float dp = 2.0f * pp[1];
+                dp = dp * (th - pp[0]);
+                const float c = -dp;
+                const float a1 = sn * d12, a3 = sn * d32;
+                float f1[3], f2[3], f3[3];
+                for (int d = 0; d < 3; ++d) {
+                    const float u12 = r12[d] / d12, u32 = r32[d] / d32;
+                    float g1 = u12 * cs - u32;
+                    g1 = g1 / a1;
+                    float g3 = u32 * cs - u12;
+                    g3 = g3 / a3;
+                    f1[d] = c * g1;
+                    f3[d] = c * g3;
+                    f2[d] = -f1[d] - f3[d];
+                }
+                /* the vertex takes minus the sum of the converted end forces, so the
+                 * term's fixed-point forces sum to zero exactly */
+                int64_t k1[3], k3[3], k2[3];
+                to_fix(f1, k1);
+                to_fix(f3, k3);
+                for (int d = 0; d < 3; ++d) k2[d] = (int64_t)(0ull - (uint64_t)k1[d] - (uint64_t)k3[d]);
+                (void)f2;
+                add_k(fix_out, at[0], k1);
+                add_k(fix_out, at[1], k2);
+                add_k(fix_out, at[2], k3);
+            }
+        } else {
+            /* torsion, Bonded/calcTorsion_v2.v:70-274 potential k (1 + cos(n phi + d)):
+             * rij = xi - xj, rkj = xk - xj, rkl = xk - xl, m = rij x rkj, v = rkj x rkl,
+             * cos phi = m.v / (|m||v|), sin phi = |rkj| (rij . v) / (|m||v|),
+             * dU/dphi = -k n sin(n phi + d); fi = -dU/dphi |rkj| / |m|^2 m,
+             * fl = dU/dphi |rkj| / |v|^2 v, s = (rij.rkj / |rkj|^2) fi - (rkl.rkj / |rkj|^2) fl,
+             * Fi += fi, Fj -= fi - s, Fk -= fl + s, Fl += fl */
+            const float *x3 = pos4 + 4 * (size_t)at[2], *x4 = pos4 + 4 * (size_t)at[3];
+            float rij[3], rkj[3], rkl[3], m[3], v[3];
+            for (int d = 0; d < 3; ++d) {
+                rij[d] = img_f(x1[d] - x2[d], box[d]);
+                rkj[d] = img_f(x3[d] - x2[d], box[d]);
+                rkl[d] = img_f(x3[d] - x4[d], box[d]);
+            }
+            cross3f(rij, rkj, m);
+            cross3f(rkj, rkl, v);
+            const float m2 = dot3f(m, m), v2 = dot3f(v, v);
+            if (m2 > 1e-12f && v2 > 1e-12f) {
+                const float kj2 = dot3f(rkj, rkj);
+                const float kj = sqrtf(kj2);
+                const float mv = sqrtf(m2) * sqrtf(v2);
+                const float cp = dot3f(m, v) / mv;
+                float sp = kj * dot3f(rij, v);
+                sp = sp / mv;
+                const int nper = (int)pp[1];
+                float cn = cp, sn = sp;
+                for (int k = 1; k < nper; ++k) {
+                    const float c2 = cn * cp - sn * sp;
+                    const float s2 = sn * cp + cn * sp;
+                    cn = c2;
+                    sn = s2;
+                }
+                float sarg = sn * pp[2] + cn * pp[3];      /* sin(n phi + d) */
+                float carg = cn * pp[2] - sn * pp[3];      /* cos(n phi + d) */
+                float ddphi = -pp[0] * pp[1];
+                ddphi = ddphi * sarg;
+                float a = -ddphi * kj;
+                a = a / m2;
+                float b = ddphi * kj;
+                b = b / v2;
+                const float p = dot3f(rij, rkj) / kj2, q = dot3f(rkl, rkj) / kj2;
+                float fi[3], fl[3], fj[3], fk[3];
+                for (int d = 0; d < 3; ++d) {
+                    fi[d] = a * m[d];
+                    fl[d] = b * v[d];
+                    const float sv = p * fi[d] - q * fl[d];
+                    fj[d] = -(fi[d] - sv);
+                    fk[d] = -(fl[d] + sv);
+                }
+                /* k takes minus the sum of the other three converted forces */
+                int64_t qi[3], qj[3], qk[3], ql[3];
+                to_fix(fi, qi);
+                to_fix(fj, qj);
+                to_fix(fl, ql);
+                for (int d = 0; d < 3; ++d)
+                    qk[d] = (int64_t)(0ull - (uint64_t)qi[d] - (uint64_t)qj[d] - (uint64_t)ql[d]);
+                (void)fk;
+                add_k(fix_out, at[0], qi);
+                add_k(fix_out, at[1], qj);
+                add_k(fix_out, at[2], qk);
+                add_k(fix_out, at[3], ql);
+                e = (double)pp[0] / 10.0 * (1.0 + (double)carg);
+            }
+        }
+        if (term_energy) term_energy[t] = e;
+    }
+}
+
+/* ---- FP64 reference of the bonded potentials ---- */
+static inline double img_d(double d, double L)
+{
+    return d - L * floor(d / L + 0.5);
+}
+static inline double dot3d(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3d(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double or_bonded_fp64(int n, const double *xyz, const double box[3], int nterms,
+                      const int32_t *atoms, const double *params, double *f_out)
+{
+    (void)n;
+    double etot = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+        const int32_t *at = atoms + 4 * (size_t)t;
+        const double *pp = params + 4 * (size_t)t;
+        const double *x1 = xyz + 3 * (size_t)at[0], *x2 = xyz + 3 * (size_t)at[1];
+        if (at[2] < 0) {
+            double r12[3];
+            for (int d = 0; d < 3; ++d) r12[d] = img_d(x1[d] - x2[d], box[d]);
+            const double r = sqrt(dot3d(r12, r12));
+            const double dp = 2.0 * pp[1] * (r - pp[0]);
+            for (int d = 0; d < 3; ++d) {
+                f_out[3 * at[0] + d] += -dp / r * r12[d];
+                f_out[3 * at[1] + d] -= -dp / r * r12[d];
+            }
+            etot += pp[1] * (r - pp[0]) * (r - pp[0]);
+        } else if (at[3] < 0) {
+            const double *x3 = xyz + 3 * (size_t)at[2];
+            double r12[3], r32[3], cr[3];
+            for (int d = 0; d < 3; ++d) {
+                r12[d] = img_d(x1[d] - x2[d], box[d]);
+                r32[d] = img_d(x3[d] - x2[d], box[d]);
+            }
+            const double d12 = sqrt(dot3d(r12, r12)), d32 = sqrt(dot3d(r32, r32));
+            cross3d(r12, r32, cr);
+            const double y = sqrt(dot3d(cr, cr)), xx = dot3d(r12, r32);
+            const double th = atan2(y, xx);
+            const double cs = cos(th), sn = sin(th);
+            const double dp = 2.0 * pp[1] * (th - pp[0]);
+            etot += pp[1] * (th - pp[0]) * (th - pp[0]);
+            if (sn > 1e-12) {
+                for (int d = 0; d < 3; ++d) {
+                    const double g1 = (r12[d] / d12 * cs - r32[d] / d32) / (sn * d12);
+                    const double g3 = (r32[d] / d32 * cs - r12[d] / d12) / (sn * d32);
+                    f_out[3 * at[0] + d] += -dp * g1;
+                    f_out[3 * at[2] + d] += -dp * g3;
+                    f_out[3 * at[1] + d] += dp * g1 + dp * g3;
+                }
+            }
+        } else {
+            const double *x3 = xyz + 3 * (size_t)at[2], *x4 = xyz + 3 * (size_t)at[3];
+            double rij[3], rkj[3], rkl[3], m[3], v[3];
+            for (int d = 0; d < 3; ++d) {
+                rij[d] = img_d(x1[d] - x2[d], box[d]);
+                rkj[d] = img_d(x3[d] - x2[d], box[d]);
+                rkl[d] = img_d(x3[d] - x4[d], box[d]);
+            }
+            cross3d(rij, rkj, m);
+            cross3d(rkj, rkl, v);
+            const double m2 = dot3d(m, m), v2 = dot3d(v, v), kj2 = dot3d(rkj, rkj);
+            const double kj = sqrt(kj2);
+            const double phi = atan2(kj * dot3d(rij, v), dot3d(m, v));
+            const double nper = pp[1], ph = pp[2];
+            etot += pp[0] * (1.0 + cos(nper * phi + ph));
+            const double ddphi = -pp[0] * nper * sin(nper * phi + ph);
+            const double p = dot3d(rij, rkj) / kj2, q = dot3d(rkl, rkj) / kj2;
+            for (int d = 0; d < 3; ++d) {
+                const double fi = -ddphi * kj / m2 * m[d], fl = ddphi * kj / v2 * v[d];
+                const double sv = p * fi - q * fl;
+                f_out[3 * at[0] + d] += fi;
+                f_out[3 * at[1] + d] -= fi - sv;
+                f_out[3 * at[2] + d] -= fl + sv;
+                f_out[3 * at[3] + d] += fl;
+            }
+        }
+    }
+    return etot;
 }

@@ -113,7 +113,8 @@ typedef struct {
     const float *tab;         /* interleaved [nseg*256][6] = c0_14 c1_14 c0_8 c1_8 c0_3 c1_3 */
     int exp_offset, nseg;
     int mode;
-    const uint8_t *excl_mask; /* optional per-atom exclusion data (NULL = none) */
+    const uint32_t *excl;     /* optional exclusion windows, one per atom (NULL = none):
+                                 bit d of excl[i] excludes the pair (i, i + d - 16) */
 } or_rl_system;
 
 /* Cell index of one coordinate: (int)(x * inv_edge) clamped to [0, nc-1], with
@@ -141,7 +142,8 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out);
 int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *q,
                        int ntypes, const double *eps_ij, const double *sig_ij,
                        double ke, int coulomb, const double box[3], double rc,
-                       int use_cells, double *f_out, double *energy, double *virial);
+                       int use_cells, double *f_out, double *energy, double *virial,
+                       const uint32_t *excl);
 
 /* FP64 evaluation of the same first-order table model (error measure (i)). */
 int64_t or_table_fp64(const or_rl_system *s, double *f_out);
@@ -162,6 +164,31 @@ void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,
  * F = fix * 2^-32, s = min(gamma, dmax / |F|), x' = wrap(x + (float)(F * s)). */
 void or_minimize_update(int n, float *pos4, const int64_t *fix, double gamma, double dmax,
                         const float box[3]);
+
+/* ---- bonded terms (BASELINE workload 5) ---------------------------------------- */
+/* One term per row of atoms[nterms][4] (input indices, -1 pads): a bond (i, j, -1, -1),
+ * an angle (i, j, k, -1) with vertex j, or a torsion (i, j, k, l).  params[nterms][4],
+ * forces in kJ/mol/nm from positions in A (the constants carry the factor 10):
+ *   bond    (r0, 10 k, -, -):          U = k (r - r0)^2        Bonded/calc_bond_v2.v:83-110
+ *   angle   (theta0, 10 k, -, -):      U = k (theta - theta0)^2  Bonded/calc_angle_v2.v:113-147
+ *   torsion (10 k, n, cos d, sin d):   U = k (1 + cos(n phi + d))  Bonded/calcTorsion_v2.v:70-274
+ * Vectors between term atoms take the minimum image of the box.  theta uses or_atan2f
+ * (Taylor series of arctan after a reduction by pi/6); phi enters only through
+ * cos(n phi) and sin(n phi), built by the angle-addition recurrence from cos phi and
+ * sin phi.  Torsion forces follow the projection form of Bekker (1996) / Blondel and
+ * Karplus (1996), with no division by sin phi. */
+float or_atan2f(float y, float x);
+/* FP32 forces, each atom's share converted by or_fix and added into fix_out (n x 3,
+ * accumulated onto its contents).  term_energy (may be NULL): FP64 energy per term from
+ * the same FP32 positions, in kJ/mol (k taken as params / 10). */
+void or_bonded_fix(int n, const float *pos4, const float box[3], int nterms,
+                   const int32_t *atoms, const float *params, int64_t *fix_out,
+                   double *term_energy);
+/* The same potentials in FP64 (forces in kJ/mol/A, k in kJ/mol/A^2 etc. without the
+ * factor 10, torsion phase in params[2] as an angle): forces added into f_out, total
+ * energy returned.  Used to check the FP32 version and, by finite differences, itself. */
+double or_bonded_fp64(int n, const double *xyz, const double box[3], int nterms,
+                      const int32_t *atoms, const double *params, double *f_out);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,7 @@ class RLSystem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("box", ctypes.c_float * 3), ("ncell", ctypes.c_int * 3),
                 ("rc2", ctypes.c_float), ("pos4", F), ("type", I), ("ntypes", ctypes.c_int),
                 ("pair_a", F), ("pair_b", F), ("tab", F), ("exp_offset", ctypes.c_int),
-                ("nseg", ctypes.c_int), ("mode", ctypes.c_int), ("excl_mask", ctypes.c_void_p)]
+                ("nseg", ctypes.c_int), ("mode", ctypes.c_int), ("excl", ctypes.c_void_p)]
 
 
 def build():
@@ -71,8 +71,15 @@ def lib():
         L.or_table_fp64.argtypes = [ctypes.POINTER(RLSystem), D]
         L.or_table_fp64.restype = ctypes.c_int64
         L.or_fp64_direct.argtypes = [ctypes.c_int, D, I, D, ctypes.c_int, D, D, ctypes.c_double,
-                                     ctypes.c_int, D, ctypes.c_double, ctypes.c_int, D, D, D]
+                                     ctypes.c_int, D, ctypes.c_double, ctypes.c_int, D, D, D,
+                                     ctypes.c_void_p]
         L.or_fp64_direct.restype = ctypes.c_int64
+        L.or_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.or_atan2f.restype = ctypes.c_float
+        L.or_bonded_fix.argtypes = [ctypes.c_int, F, F, ctypes.c_int, I, F, I64, D]
+        L.or_bonded_fix.restype = None
+        L.or_bonded_fp64.argtypes = [ctypes.c_int, D, D, ctypes.c_int, I, D, D]
+        L.or_bonded_fp64.restype = ctypes.c_double
         L.or_motion_update.argtypes = [ctypes.c_int, F, F, I64, I, D, D, ctypes.c_float, F,
                                        ctypes.c_int, ctypes.c_int, D]
         L.or_motion_update.restype = None
@@ -201,7 +208,7 @@ class RL:
     """An or_rl_system with owned arrays."""
 
     def __init__(self, pos_xyz, box, ncell, rc, types, eps, sigma, tab6, charge=None,
-                 coulomb_k=0.0, mode=0, exp_offset=121, nseg=14):
+                 coulomb_k=0.0, mode=0, exp_offset=121, nseg=14, excl=None):
         n = pos_xyz.shape[0]
         self.pos4 = np.zeros((n, 4), np.float32)
         self.pos4[:, :3] = pos_xyz
@@ -221,7 +228,8 @@ class RL:
         s.ntypes = len(eps)
         s.pair_a, s.pair_b, s.tab = _p(self.pa, F), _p(self.pb, F), _p(self.tab, F)
         s.exp_offset, s.nseg, s.mode = exp_offset, nseg, mode
-        s.excl_mask = None
+        self.excl = None if excl is None else np.ascontiguousarray(excl, np.uint32)
+        s.excl = None if self.excl is None else self.excl.ctypes.data_as(ctypes.c_void_p)
         self.s = s
         self.n = n
 
@@ -236,8 +244,9 @@ class RL:
         return out, npairs
 
 
-def fp64_direct(xyz, types, eps, sigma, box, rc, charge=None, ke=0.0, use_cells=True):
-    """Analytic FP64 forces (kJ/mol/A), LJ with Lorentz-Berthelot + cutoff Coulomb."""
+def fp64_direct(xyz, types, eps, sigma, box, rc, charge=None, ke=0.0, use_cells=True, excl=None):
+    """Analytic FP64 forces (kJ/mol/A), LJ with Lorentz-Berthelot + cutoff Coulomb;
+    excl: optional exclusion windows (bit d of excl[i] excludes the pair (i, i + d - 16))."""
     xyz = np.ascontiguousarray(xyz, np.float64)
     n = xyz.shape[0]
     nt = len(eps)
@@ -256,7 +265,9 @@ def fp64_direct(xyz, types, eps, sigma, box, rc, charge=None, ke=0.0, use_cells=
     npairs = lib().or_fp64_direct(n, _p(xyz, D), _p(t, I), None if q is None else _p(q, D), nt,
                                   _p(e, D), _p(s, D), float(ke), 1 if (q is not None and ke) else 0,
                                   _p(bx, D), float(rc), 1 if use_cells else 0, _p(f, D),
-                                  ctypes.byref(en), _p(vir, D))
+                                  ctypes.byref(en), _p(vir, D),
+                                  None if excl is None else
+                                  np.ascontiguousarray(excl, np.uint32).ctypes.data_as(ctypes.c_void_p))
     return f, en.value, vir.reshape(3, 3), npairs
 
 
@@ -299,3 +310,52 @@ def minimize_update(pos4, fixf, gamma, dmax, box):
     lib().or_minimize_update(pos4.shape[0], _p(pos4, F), _p(fixf, I64), float(gamma),
                              float(dmax), _p(bx, F))
     return pos4
+
+
+# ---------------------------------------------------------------------------- bonded
+def atan2f(y, x):
+    return lib().or_atan2f(float(np.float32(y)), float(np.float32(x)))
+
+
+def device_bonded_params(atoms, params):
+    """User-level term parameters -> the FP32 constants the kernels use, with the same
+    host arithmetic as the library: bond/angle (x0, 10*k), torsion (10*k, n, cos d, sin d)."""
+    atoms = np.ascontiguousarray(atoms, np.int32).reshape(-1, 4)
+    params = np.ascontiguousarray(params, np.float32).reshape(-1, 4)
+    out = np.zeros_like(params)
+    tors = atoms[:, 3] >= 0
+    out[~tors, 0] = params[~tors, 0]
+    out[~tors, 1] = np.float32(10.0) * params[~tors, 1]
+    out[tors, 0] = np.float32(10.0) * params[tors, 0]
+    out[tors, 1] = params[tors, 1]
+    ph = params[tors, 2].astype(np.float64)
+    out[tors, 2] = np.cos(ph).astype(np.float32)
+    out[tors, 3] = np.sin(ph).astype(np.float32)
+    return out
+
+
+def bonded_fix(pos4, box, atoms, params):
+    """int64 bonded forces (2^-32 kJ/mol/nm) and FP64 per-term energies (kJ/mol), from
+    user-level parameters (converted by device_bonded_params)."""
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    n = pos4.shape[0]
+    atoms = np.ascontiguousarray(atoms, np.int32).reshape(-1, 4)
+    dp = device_bonded_params(atoms, params)
+    bx = np.ascontiguousarray(box, np.float32)
+    fix = np.zeros((n, 3), np.int64)
+    e = np.zeros(atoms.shape[0])
+    lib().or_bonded_fix(n, _p(pos4, F), _p(bx, F), atoms.shape[0], _p(atoms, I), _p(dp, F),
+                        _p(fix, I64), _p(e, D))
+    return fix, e
+
+
+def bonded_fp64(xyz, box, atoms, params):
+    """FP64 bonded forces (kJ/mol/A) and total energy (kJ/mol) from user-level parameters."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    atoms = np.ascontiguousarray(atoms, np.int32).reshape(-1, 4)
+    pp = np.ascontiguousarray(params, np.float64).reshape(-1, 4)
+    bx = np.ascontiguousarray(box, np.float64)
+    f = np.zeros((xyz.shape[0], 3))
+    e = lib().or_bonded_fp64(xyz.shape[0], _p(xyz, D), _p(bx, D), atoms.shape[0], _p(atoms, I),
+                             _p(pp, D), _p(f, D))
+    return f, e

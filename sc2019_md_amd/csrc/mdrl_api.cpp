@@ -189,6 +189,12 @@ struct md_rl_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps, one per buffer parity
     float4 *d_tmp4 = nullptr;         // n: full-step velocities for read-back
     md_rl_table *own_table = nullptr;
+    // staged bonded terms (batch-wide input indices, kernel constants) and exclusion
+    // windows, per system
+    std::vector<std::vector<int4>> t_atoms;
+    std::vector<std::vector<float4>> t_params;
+    std::vector<std::vector<uint32_t>> t_excl;
+    size_t cap_terms = 0;
 };
 
 static void drop_graphs(md_rl_ctx *c)
@@ -203,7 +209,8 @@ static void free_ctx_device(md_rl_ctx *c)
     void *ptrs[] = {s.geom, s.cell_sys, s.slot_sys, s.tasks, s.pos, s.pos2, s.vel, s.vel2, s.type,
                     s.type2, s.orig, s.orig2, s.cell, s.perm, s.cell_start, s.cell_next,
                     s.cell_fill, s.fix, s.f32, s.tab, s.etab, s.pair_ab, s.pair_ab_e, s.hk,
-                    s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4};
+                    s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4, s.excl,
+                    s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
@@ -435,6 +442,9 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     c->h_v.assign(3 * (size_t)n, 0.0f);
     c->h_type.assign(n, 0);
     c->h_q.assign(n, 0.0f);
+    c->t_atoms.resize(nsys);
+    c->t_params.resize(nsys);
+    c->t_excl.resize(nsys);
     *out = c;
     return MDRL_OK;
 }
@@ -464,6 +474,61 @@ extern "C" int md_rl_system_size(const md_rl_ctx *c, int sys, int32_t *n, float 
         if (box) box[d] = G.box[d];
         if (ncell) ncell[d] = G.nc[d];
     }
+    return MDRL_OK;
+}
+
+// Bonded terms and exclusion windows of every system onto the device (before binning,
+// which fills slot_of).
+static int upload_topology(md_rl_ctx *c)
+{
+    DeviceSystem &s = c->s;
+    std::vector<int4> atoms;
+    std::vector<float4> params;
+    std::vector<int> off(1, 0);
+    bool any_excl = false;
+    for (int r = 0; r < s.nsys; ++r) {
+        atoms.insert(atoms.end(), c->t_atoms[r].begin(), c->t_atoms[r].end());
+        params.insert(params.end(), c->t_params[r].begin(), c->t_params[r].end());
+        off.push_back((int)atoms.size());
+        any_excl = any_excl || !c->t_excl[r].empty();
+    }
+    int rc;
+    hipError_t e;
+    const bool need_slots = !atoms.empty();
+    if (need_slots && !s.slot_of && (rc = dalloc(&s.slot_of, (size_t)s.n))) return rc;
+    if (!need_slots && s.slot_of) { (void)hipFree(s.slot_of); s.slot_of = nullptr; }
+    if (atoms.size() > c->cap_terms) {
+        for (void *p : {(void *)s.term_atoms, (void *)s.term_params, (void *)s.term_e})
+            if (p) (void)hipFree(p);
+        s.term_atoms = nullptr; s.term_params = nullptr; s.term_e = nullptr;
+        if ((rc = dalloc(&s.term_atoms, atoms.size())) || (rc = dalloc(&s.term_params, atoms.size())) ||
+            (rc = dalloc(&s.term_e, atoms.size())))
+            return rc;
+        c->cap_terms = atoms.size();
+    }
+    if (!s.term_off && (rc = dalloc(&s.term_off, (size_t)s.nsys + 1))) return rc;
+    s.nterms = (int)atoms.size();
+    if (!atoms.empty() &&
+        ((e = hipMemcpyAsync(s.term_atoms, atoms.data(), sizeof(int4) * atoms.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+         (e = hipMemcpyAsync(s.term_params, params.data(), sizeof(float4) * params.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess))
+        return hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipMemcpyAsync(s.term_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
+    if (any_excl) {
+        std::vector<uint32_t> m((size_t)s.n, 0u);
+        for (int r = 0; r < s.nsys; ++r)
+            if (!c->t_excl[r].empty())
+                std::copy(c->t_excl[r].begin(), c->t_excl[r].end(), m.begin() + s.h_geom[r].atom_off);
+        if (!s.excl && (rc = dalloc(&s.excl, (size_t)s.n))) return rc;
+        if ((e = hipMemcpyAsync(s.excl, m.data(), sizeof(uint32_t) * m.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync");
+        if (!schedule_supported(s, s.sched)) { s.sched.apw = 64; s.sched.tile = 256; }
+    } else if (s.excl) {
+        (void)hipFree(s.excl);
+        s.excl = nullptr;
+    }
+    // the staging buffers must outlive the asynchronous copies
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return MDRL_OK;
 }
 
@@ -502,6 +567,7 @@ static int upload(md_rl_ctx *c)
     if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * s.ncells, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync");
     int rc;
+    if ((rc = upload_topology(c))) return rc;
     if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream))) return rc;
     s.first_step = true;
     drop_graphs(c);
@@ -603,6 +669,90 @@ extern "C" int md_rl_set_particles(md_rl_ctx *c, const float *particle, int n)
     int rc = stage_system(c, 0, particle, nullptr, nullptr, false);
     if (rc) return rc;
     return upload(c);
+}
+
+// Keeps the device state of every system when part of the input is staged again.
+static int restage(md_rl_ctx *c)
+{
+    if (c->have_particles && !c->dirty) {
+        int rc = pull_state(c);
+        if (rc) return rc;
+    }
+    c->dirty = true;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t *atoms,
+                                const float *params)
+{
+    if (!c || sys < 0 || sys >= c->s.nsys || nterms < 0 || (nterms > 0 && (!atoms || !params))) {
+        set_error("md_rl_set_bonded: bad argument");
+        return MDRL_EINVAL;
+    }
+    const SysGeom &G = c->s.h_geom[sys];
+    std::vector<int4> ta((size_t)nterms);
+    std::vector<float4> tp((size_t)nterms);
+    for (int t = 0; t < nterms; ++t) {
+        const int32_t *a = atoms + 4 * (size_t)t;
+        const float *q = params + 4 * (size_t)t;
+        const int na = a[2] < 0 ? 2 : a[3] < 0 ? 3 : 4;
+        bool ok = (na == 2 ? a[3] < 0 : true);
+        for (int k = 0; k < na; ++k) ok = ok && a[k] >= 0 && a[k] < G.n;
+        for (int k = 0; k < na; ++k)
+            for (int l = k + 1; l < na; ++l) ok = ok && a[k] != a[l];
+        for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(q[k]);
+        if (na == 4) ok = ok && q[1] >= 1.0f && q[1] <= 6.0f && q[1] == floorf(q[1]);
+        if (!ok) {
+            set_error("md_rl_set_bonded: term %d of system %d is invalid", t, sys);
+            return MDRL_EINVAL;
+        }
+        int g[4];
+        for (int k = 0; k < 4; ++k) g[k] = k < na ? G.atom_off + a[k] : -1;
+        ta[t] = make_int4(g[0], g[1], g[2], g[3]);
+        if (na == 4) {
+            // k (1 + cos(n phi + phase)): (10 k, n, cos phase, sin phase)
+            const double ph = (double)q[2];
+            tp[t] = make_float4(10.0f * q[0], q[1], (float)cos(ph), (float)sin(ph));
+        } else {
+            // k (x - x0)^2: (x0, 10 k)
+            tp[t] = make_float4(q[0], 10.0f * q[1], 0.0f, 0.0f);
+        }
+    }
+    int rc = restage(c);
+    if (rc) return rc;
+    c->t_atoms[sys].swap(ta);
+    c->t_params[sys].swap(tp);
+    return (c->s.nsys == 1 && c->have_particles) ? upload(c) : MDRL_OK;
+}
+
+extern "C" int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask)
+{
+    if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_set_exclusions: bad argument"); return MDRL_EINVAL; }
+    if (mask && (c->s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC))) {
+        set_error("md_rl_set_exclusions: exclusions need the default int64 table path");
+        return MDRL_EINVAL;
+    }
+    const SysGeom &G = c->s.h_geom[sys];
+    std::vector<uint32_t> m;
+    if (mask) {
+        m.assign(mask, mask + G.n);
+        for (int i = 0; i < G.n; ++i) {
+            if (m[i] & (1u << 16)) { set_error("md_rl_set_exclusions: atom %d excludes itself", i); return MDRL_EINVAL; }
+            for (int d = 0; d < 32; ++d) {
+                if (!((m[i] >> d) & 1u)) continue;
+                const int j = i + d - 16;
+                if (j < 0 || j >= G.n || !((m[j] >> (32 - d)) & 1u)) {
+                    set_error("md_rl_set_exclusions: window of atom %d names atom %d, out of range or "
+                              "not excluding it back", i, j);
+                    return MDRL_EINVAL;
+                }
+            }
+        }
+    }
+    int rc = restage(c);
+    if (rc) return rc;
+    c->t_excl[sys].swap(m);
+    return (c->s.nsys == 1 && c->have_particles) ? upload(c) : MDRL_OK;
 }
 
 extern "C" int md_rl_commit(md_rl_ctx *c)
@@ -855,7 +1005,7 @@ extern "C" int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke
     if (rc) return rc;
     const double *r = red.data() + (size_t)kRedStride * sys;
     if (ke) *ke = r[0];
-    if (pe) *pe = r[1];
+    if (pe) *pe = r[1] + r[11];
     if (virial) for (int k = 0; k < 9; ++k) virial[k] = r[2 + k];
     return MDRL_OK;
 }
@@ -866,11 +1016,11 @@ extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[
     std::vector<double> red;
     int rc = energies(c, red);
     if (rc) return rc;
-    double t[11] = {0};
+    double t[12] = {0};
     for (int r = 0; r < c->s.nsys; ++r)
-        for (int k = 0; k < 11; ++k) t[k] += red[(size_t)kRedStride * r + k];
+        for (int k = 0; k < 12; ++k) t[k] += red[(size_t)kRedStride * r + k];
     if (ke) *ke = t[0];
-    if (pe) *pe = t[1];
+    if (pe) *pe = t[1] + t[11];
     if (virial) for (int k = 0; k < 9; ++k) virial[k] = t[2 + k];
     return MDRL_OK;
 }

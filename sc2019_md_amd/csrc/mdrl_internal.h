@@ -66,7 +66,7 @@ constexpr int kQueues = 8;
 constexpr int kQueueWord = 8 + 2 * kStatSlots;
 constexpr int kCounterWords = kQueueWord + kQueues;
 
-constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], -
+constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], bonded PE
 
 // Device state of a batch of systems.
 struct DeviceSystem {
@@ -107,6 +107,14 @@ struct DeviceSystem {
     double *cell_red = nullptr;               // ncells x 10: PE + virial partials per home cell
     double *atom_ke = nullptr;                // n: kinetic energy per slot
     double *red = nullptr;                    // nsys x kRedStride
+    // bonded terms and nonbonded exclusions (batch-wide input indices)
+    uint32_t *excl = nullptr;                 // n exclusion windows by input index, or NULL
+    int *slot_of = nullptr;                   // n: input index -> slot, kept by the rebin
+    int nterms = 0;
+    int4 *term_atoms = nullptr;               // nterms: bond (i,j,-1,-1), angle (i,j,k,-1), torsion
+    float4 *term_params = nullptr;            // nterms: kernel constants (factor 10 folded in)
+    double *term_e = nullptr;                 // nterms: FP64 energy per term (on request)
+    int *term_off = nullptr;                  // nsys + 1: terms of each system
     int exp_offset = 121, nseg = 14;
     TableWindow win;
     Schedule sched;

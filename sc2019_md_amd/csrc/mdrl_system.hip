@@ -67,9 +67,13 @@ struct ForceArgs {
     unsigned long long *counters;
     int ngroups;
     int gstart[8];
+    const int *orig;          // slot -> input index (exclusions)
+    const uint32_t *excl;     // exclusion windows by input index, or NULL
 };
 
-template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ>
+// EXC: nonbonded exclusions; a queued pair whose neighbour lies in the home atom's
+// exclusion window contributes no force and is not counted.
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_rl_halfshell(ForceArgs a)
 {
     constexpr int STR = COUL ? 6 : 4;
@@ -86,6 +90,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float2 s_ab[64];
     __shared__ int s_task;
     __shared__ unsigned long long s_stat[NWAVE][2];
+    __shared__ int s_io[EXC ? TI : 1];
+    __shared__ uint32_t s_im[EXC ? TI : 1];
+    __shared__ int s_jo[EXC ? TJ : 1];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -107,7 +114,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
     unsigned *q = s_q[wave];
     unsigned flag = 0;
-    unsigned long long npairs = 0, ncand = 0;
+    unsigned long long npairs = 0, ncand = 0, nexcl = 0;
     const unsigned rc2_bits = __float_as_uint(a.rc2);
 
     for (int it = 0;; ++it) {
@@ -172,6 +179,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             for (int k = tid; k < mi; k += NTHR) {
                 s_i[k] = a.pos[hstart + itile + k];
                 s_it[k] = a.type[hstart + itile + k];
+                if (EXC) {
+                    const int io = a.orig[hstart + itile + k];
+                    s_io[k] = io;
+                    s_im[k] = a.excl[io];
+                }
             }
             for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
             if (tid < cur.mj) {
@@ -179,6 +191,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 if (!PAR) { p.x = p.x + cur.sx; p.y = p.y + cur.sy; p.z = p.z + cur.sz; }
                 s_j[tid] = p;
                 s_jt[tid] = a.type[cur.base + tid];
+                if (EXC) s_jo[tid] = a.orig[cur.base + tid];
             }
             for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
             lds_barrier();
@@ -186,10 +199,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             while (cur.o < oend) {
                 const NbrTile nxt = find(cur.o, cur.jt + TJ);
                 float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
-                int tf = 0;
+                int tf = 0, of = 0;
                 if (tid < nxt.mj) {
                     pf = a.pos[nxt.base + tid];
                     tf = a.type[nxt.base + tid];
+                    if (EXC) of = a.orig[nxt.base + tid];
                 }
                 const bool self = (cur.o == 0);
                 const int jtile = cur.jt, mj = cur.mj;
@@ -203,8 +217,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
                     const float2 ab = s_ab[e >> 16];
                     const float qq = pa.w * pb.w;
-                    const float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
-                                                                     a.off, a.nseg, a.win_lo, a.nwin);
+                    float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
+                                                               a.off, a.nseg, a.win_lo, a.nwin);
+                    if (EXC) {
+                        const int d = s_jo[jb] - s_io[ia] + 16;
+                        const bool ex = (unsigned)d < 32u && ((s_im[ia] >> d) & 1u);
+                        nexcl += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(ex));
+                        F = ex ? 0.0f : F;
+                    }
                     const float fx = F * dx, fy = F * dy, fz = F * dz;
                     if (F32) {
                         atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
@@ -313,6 +333,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     if (!PAR) { pf.x = pf.x + nxt.sx; pf.y = pf.y + nxt.sy; pf.z = pf.z + nxt.sz; }
                     s_j[tid] = pf;
                     s_jt[tid] = tf;
+                    if (EXC) s_jo[tid] = of;
                 }
                 lds_barrier();
                 cur = nxt;
@@ -335,7 +356,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     }
     // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
-        s_stat[wave][0] = npairs;
+        s_stat[wave][0] = npairs - nexcl;
         s_stat[wave][1] = ncand * 64ull;
     }
     lds_barrier();
@@ -346,6 +367,250 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     }
     if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------
+// Bonded terms (Bonded/Bonded_Top.v:202-380 streams bonds, angles and torsions through
+// calc_bond_v2.v, calc_angle_v2.v and calcTorsion_v2.v): one thread per term, FP32 forces
+// in kJ/mol/nm (the constants carry the factor 10), added into the same fixed-point
+// buffer as the range-limited forces.  Terms name atoms by input index; slot_of maps
+// them to the current slots.  The arithmetic is written operation by operation (the
+// build does not contract), so the oracle restates it bit for bit.
+// ---------------------------------------------------------------------------------
+
+struct BondedArgs {
+    const float4 *pos;
+    const int *slot_of;
+    const int *slot_sys;
+    const SysGeom *geom;
+    const int4 *atoms;
+    const float4 *params;
+    int nterms;
+    long long *fix;
+    float *f32;
+    double *term_e;
+    unsigned long long *counters;
+};
+
+__device__ __forceinline__ float img_f(float d, float L)
+{
+    const float h = 0.5f * L;
+    if (d > h) return d - L;
+    if (d < -h) return d + L;
+    return d;
+}
+__device__ __forceinline__ float dot3f(const float *a, const float *b)
+{
+    float r = a[0] * b[0] + a[1] * b[1];
+    return r + a[2] * b[2];
+}
+__device__ __forceinline__ void cross3f(const float *a, const float *b, float *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// atan2 for the bond angle: reduce to a = min/max in [0, 1]; above tan(pi/12) use
+// atan a = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3)); then the Taylor series of atan to
+// t^11 (|t| <= 0.268, next term < 3e-9).
+__device__ __forceinline__ float angle_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    const float a = mn / mx;
+    float t = a, off = 0.0f;
+    if (a > 0.26794919f) {
+        const float num = a * 1.7320508f - 1.0f;
+        const float den = a + 1.7320508f;
+        t = num / den;
+        off = 0.52359878f;
+    }
+    const float t2 = t * t;
+    float p = -0.09090909f;
+    p = p * t2 + 0.11111111f;
+    p = p * t2 - 0.14285715f;
+    p = p * t2 + 0.2f;
+    p = p * t2 - 0.33333334f;
+    p = p * t2 + 1.0f;
+    float r = p * t;
+    r = off + r;
+    if (ay > ax) r = 1.5707964f - r;
+    if (x < 0.0f) r = 3.1415927f - r;
+    return y < 0.0f ? -r : r;
+}
+
+// The forces of a term's atoms, added atomically.  In fixed point the last atom takes
+// minus the sum of the others' converted forces, so each term's forces sum to zero exactly.
+template <int NA>
+__device__ __forceinline__ void bonded_add(const BondedArgs &a, const int *slot, const float (*f)[3],
+                                           unsigned &flag)
+{
+    if (a.f32) {
+        for (int k = 0; k < NA; ++k)
+            for (int d = 0; d < 3; ++d) atomic_add_agent(&a.f32[4 * (size_t)slot[k] + d], f[k][d]);
+        return;
+    }
+    unsigned long long sum[3] = {0ull, 0ull, 0ull};
+    for (int k = 0; k < NA; ++k) {
+        long long q[3];
+        if (k < NA - 1) {
+            fix3(f[k][0], f[k][1], f[k][2], q[0], q[1], q[2], flag);
+            for (int d = 0; d < 3; ++d) sum[d] += (unsigned long long)q[d];
+        } else {
+            for (int d = 0; d < 3; ++d) q[d] = (long long)(0ull - sum[d]);
+        }
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot[k]);
+        for (int d = 0; d < 3; ++d) atomic_add_agent(&dst[d], (unsigned long long)q[d]);
+    }
+}
+
+// ENERGY: write the FP64 energy of each term (from the same FP32 intermediates)
+// instead of adding forces.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nterms) return;
+    const int4 at = a.atoms[t];
+    const float4 pp = a.params[t];
+    const int s0 = a.slot_of[at.x], s1 = a.slot_of[at.y];
+    const SysGeom &G = a.geom[a.slot_sys[s0]];
+    const float4 p1 = a.pos[s0], p2 = a.pos[s1];
+    const float x1[3] = {p1.x, p1.y, p1.z}, x2[3] = {p2.x, p2.y, p2.z};
+    unsigned flag = 0;
+    double e = 0.0;
+    if (at.z < 0) {
+        // bond, Bonded/calc_bond_v2.v:83-110
+        float r12[3];
+        for (int d = 0; d < 3; ++d) r12[d] = img_f(x1[d] - x2[d], G.box[d]);
+        const float r = sqrtf(dot3f(r12, r12));
+        const float dr = r - pp.x;
+        if (ENERGY) {
+            const double ddr = (double)r - (double)pp.x;
+            e = (double)pp.y / 10.0 * ddr * ddr;
+        } else {
+            float dp = 2.0f * pp.y;
+            dp = dp * dr;
+            const float c = -dp / r;
+            float f[2][3];
+            for (int d = 0; d < 3; ++d) { f[0][d] = c * r12[d]; f[1][d] = -f[0][d]; }
+            const int sl[2] = {s0, s1};
+            bonded_add<2>(a, sl, f, flag);
+        }
+    } else if (at.w < 0) {
+        // angle, Bonded/calc_angle_v2.v:113-147 (no Urey-Bradley term)
+        const int s2 = a.slot_of[at.z];
+        const float4 p3 = a.pos[s2];
+        const float x3[3] = {p3.x, p3.y, p3.z};
+        float r12[3], r32[3], cr[3];
+        for (int d = 0; d < 3; ++d) {
+            r12[d] = img_f(x1[d] - x2[d], G.box[d]);
+            r32[d] = img_f(x3[d] - x2[d], G.box[d]);
+        }
+        const float d12 = sqrtf(dot3f(r12, r12)), d32 = sqrtf(dot3f(r32, r32));
+        cross3f(r12, r32, cr);
+        const float y = sqrtf(dot3f(cr, cr)), xx = dot3f(r12, r32);
+        const float th = angle_atan2f(y, xx);
+        if (ENERGY) {
+            const double dth = (double)th - (double)pp.x;
+            e = (double)pp.y / 10.0 * dth * dth;
+        } else {
+            const float dd = d12 * d32;
+            const float cs = xx / dd, sn = y / dd;
+            if (sn > 1e-6f) {
+                float dp = 2.0f * pp.y;
+                dp = dp * (th - pp.x);
+                const float c = -dp;
+                const float a1 = sn * d12, a3 = sn * d32;
+                // order: end 1, end 3, then the vertex (minus their sum)
+                float f[3][3];
+                for (int d = 0; d < 3; ++d) {
+                    const float u12 = r12[d] / d12, u32 = r32[d] / d32;
+                    float g1 = u12 * cs - u32;
+                    g1 = g1 / a1;
+                    float g3 = u32 * cs - u12;
+                    g3 = g3 / a3;
+                    f[0][d] = c * g1;
+                    f[1][d] = c * g3;
+                    f[2][d] = -f[0][d] - f[1][d];
+                }
+                const int sl[3] = {s0, s2, s1};
+                bonded_add<3>(a, sl, f, flag);
+            }
+        }
+    } else {
+        // torsion, Bonded/calcTorsion_v2.v:70-274: k (1 + cos(n phi + d)) with the
+        // projection-form forces (no division by sin phi)
+        const int s2 = a.slot_of[at.z], s3 = a.slot_of[at.w];
+        const float4 p3 = a.pos[s2], p4 = a.pos[s3];
+        const float x3[3] = {p3.x, p3.y, p3.z}, x4[3] = {p4.x, p4.y, p4.z};
+        float rij[3], rkj[3], rkl[3], m[3], v[3];
+        for (int d = 0; d < 3; ++d) {
+            rij[d] = img_f(x1[d] - x2[d], G.box[d]);
+            rkj[d] = img_f(x3[d] - x2[d], G.box[d]);
+            rkl[d] = img_f(x3[d] - x4[d], G.box[d]);
+        }
+        cross3f(rij, rkj, m);
+        cross3f(rkj, rkl, v);
+        const float m2 = dot3f(m, m), v2 = dot3f(v, v);
+        if (m2 > 1e-12f && v2 > 1e-12f) {
+            const float kj2 = dot3f(rkj, rkj);
+            const float kj = sqrtf(kj2);
+            const float mv = sqrtf(m2) * sqrtf(v2);
+            const float cp = dot3f(m, v) / mv;
+            float sp = kj * dot3f(rij, v);
+            sp = sp / mv;
+            const int nper = (int)pp.y;
+            float cn = cp, snn = sp;
+            for (int k = 1; k < nper; ++k) {
+                const float c2 = cn * cp - snn * sp;
+                const float s2n = snn * cp + cn * sp;
+                cn = c2;
+                snn = s2n;
+            }
+            const float sarg = snn * pp.z + cn * pp.w;
+            const float carg = cn * pp.z - snn * pp.w;
+            if (ENERGY) {
+                e = (double)pp.x / 10.0 * (1.0 + (double)carg);
+            } else {
+                float ddphi = -pp.x * pp.y;
+                ddphi = ddphi * sarg;
+                float ca = -ddphi * kj;
+                ca = ca / m2;
+                float cb = ddphi * kj;
+                cb = cb / v2;
+                const float pq = dot3f(rij, rkj) / kj2, qq = dot3f(rkl, rkj) / kj2;
+                // order: i, j, l, then k (minus their sum)
+                float f[4][3];
+                for (int d = 0; d < 3; ++d) {
+                    const float fi = ca * m[d], fl = cb * v[d];
+                    const float sv = pq * fi - qq * fl;
+                    f[0][d] = fi;
+                    f[1][d] = -(fi - sv);
+                    f[2][d] = fl;
+                    f[3][d] = -(fl + sv);
+                }
+                const int sl[4] = {s0, s1, s3, s2};
+                bonded_add<4>(a, sl, f, flag);
+            }
+        }
+    }
+    if (ENERGY) a.term_e[t] = e;
+    if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+static BondedArgs bonded_args(DeviceSystem &s)
+{
+    BondedArgs b;
+    b.pos = s.pos; b.slot_of = s.slot_of; b.slot_sys = s.slot_sys; b.geom = s.geom;
+    b.atoms = s.term_atoms; b.params = s.term_params; b.nterms = s.nterms;
+    b.fix = (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix;
+    b.f32 = (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr;
+    b.term_e = s.term_e; b.counters = s.counters;
+    return b;
 }
 
 // Offset groups: the home cell is half the work of a full neighbour cell.
@@ -362,15 +627,16 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
 {
     const bool fast = !(s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC));
     const bool groups_ok = sc.groups == 1 || sc.groups == 2 || sc.groups == 4;
-    if (!groups_ok) return false;
+    if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
+    if (s.excl) return fast && sc.apw == 64 && sc.tile == 256;   // exclusions: one variant
     if (sc.apw == 64 && sc.tile == 256) return true;
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
-template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ>
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false>
 static void launch_rl(const ForceArgs &a, dim3 grid, size_t lds, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ>), grid, dim3(NTHR), lds, st, a);
+    hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>), grid, dim3(NTHR), lds, st, a);
 }
 
 template <bool COUL>
@@ -390,6 +656,8 @@ static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, si
 int launch_force(DeviceSystem &s, hipStream_t st)
 {
     if (s.n == 0) return MDRL_OK;
+    if (s.nterms > 0)
+        hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
     ForceArgs a;
     a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
     a.geom = s.geom; a.cell_sys = s.cell_sys;
@@ -402,6 +670,7 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.ngroups = s.sched.groups;
     offset_groups(a.ngroups, a.gstart);
     a.ntasks = s.ncells * a.ngroups;
+    a.orig = s.orig; a.excl = s.excl;
     const bool f32 = s.flags & MDRL_ACC_FP32, par = s.flags & MDRL_EVAL_PARITY,
                ana = s.flags & MDRL_EVAL_ANALYTIC, coul = s.coul;
     const int str = coul ? 6 : 4;
@@ -418,6 +687,9 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     } else if (f32) {
         if (coul) launch_rl<true, true, false, false, 64, 256>(a, grid, lds, st);
         else launch_rl<false, true, false, false, 64, 256>(a, grid, lds, st);
+    } else if (s.excl) {
+        if (coul) launch_rl<true, false, false, false, 64, 256, true>(a, grid, lds, st);
+        else launch_rl<false, false, false, false, 64, 256, true>(a, grid, lds, st);
     } else {
         if (coul) launch_rl_fast<true>(a, sc, grid, lds, st);
         else launch_rl_fast<false>(a, sc, grid, lds, st);
@@ -528,7 +800,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(const int *start, int *perm, 
 
 __global__ void k_gather(int n, const int *perm, const float4 *pos, const float4 *vel,
                          const int *type, const int *orig, float4 *pos2, float4 *vel2,
-                         int *type2, int *orig2, long long *fix, float *f32)
+                         int *type2, int *orig2, long long *fix, float *f32, int *slot_of)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -536,7 +808,9 @@ __global__ void k_gather(int n, const int *perm, const float4 *pos, const float4
     pos2[s] = pos[o];
     vel2[s] = vel[o];
     type2[s] = type[o];
-    orig2[s] = orig[o];
+    const int io = orig[o];
+    orig2[s] = io;
+    if (slot_of) slot_of[io] = s;
     if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
     if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -562,7 +836,7 @@ int launch_rebin(DeviceSystem &s, hipStream_t st)
         hipLaunchKernelGGL(k_gather, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.perm, s.pos,
                            s.vel, s.type, s.orig, s.pos2, s.vel2, s.type2, s.orig2,
                            (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix,
-                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr);
+                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr, s.slot_of);
     }
     MDRL_HIP(hipGetLastError());
     std::swap(s.pos, s.pos2);
@@ -733,6 +1007,8 @@ __global__ __launch_bounds__(256) void k_energy(ForceArgs a, const float *etab, 
     for (int ia = threadIdx.x; ia < hcount; ia += blockDim.x) {
         const float4 pi = a.pos[hstart + ia];
         const int ti = a.type[hstart + ia];
+        const int oi = a.excl ? a.orig[hstart + ia] : 0;
+        const uint32_t mi = a.excl ? a.excl[oi] : 0u;
         for (int o = 0; o < 14; ++o) {
             int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
             float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -747,6 +1023,10 @@ __global__ __launch_bounds__(256) void k_energy(ForceArgs a, const float *etab, 
                 float dx, dy, dz;
                 float r2 = pair_geom<false>(pi, pj, 0.f, 0.f, 0.f, dx, dy, dz);
                 if (!(r2 > 0.0f && r2 <= a.rc2)) continue;
+                if (a.excl) {
+                    const int d = a.orig[cs + jb] - oi + 16;
+                    if ((unsigned)d < 32u && ((mi >> d) & 1u)) continue;
+                }
                 const int tj = a.type[cs + jb];
                 // ab_e = (eps sig^12, eps sig^6): E = A t12 - B t6, F = A t14 - B t8 (unit
                 // tables 4 r^-12, 4 r^-6, 48 r^-14, 24 r^-8); the scaled charges carry 10x.
@@ -806,23 +1086,27 @@ __global__ __launch_bounds__(256) void k_kinetic(int n, const float4 *vel, const
 }
 
 __global__ __launch_bounds__(256) void k_sys_reduce(const SysGeom *geom, const double *cell_red,
-                                                    const double *atom_ke, double *red)
+                                                    const double *atom_ke, const double *term_e,
+                                                    const int *term_off, double *red)
 {
-    __shared__ double s_r[11][256];
+    __shared__ double s_r[12][256];
     const SysGeom &G = geom[blockIdx.x];
     const int ncell = G.nc[0] * G.nc[1] * G.nc[2];
-    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = threadIdx.x; k < G.n; k += blockDim.x) acc[0] += atom_ke[G.atom_off + k];
     for (int c = threadIdx.x; c < ncell; c += blockDim.x)
         for (int k = 0; k < 10; ++k) acc[1 + k] += cell_red[10 * (size_t)(G.cell_off + c) + k];
-    for (int k = 0; k < 11; ++k) s_r[k][threadIdx.x] = acc[k];
+    if (term_e)
+        for (int t = term_off[blockIdx.x] + threadIdx.x; t < term_off[blockIdx.x + 1]; t += blockDim.x)
+            acc[11] += term_e[t];
+    for (int k = 0; k < 12; ++k) s_r[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off)
-            for (int k = 0; k < 11; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
+            for (int k = 0; k < 12; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x < 11) red[kRedStride * (size_t)blockIdx.x + threadIdx.x] = s_r[threadIdx.x][0];
+    if (threadIdx.x < 12) red[kRedStride * (size_t)blockIdx.x + threadIdx.x] = s_r[threadIdx.x][0];
 }
 
 int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
@@ -838,11 +1122,14 @@ int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
         a.rc2 = s.rc2;
         a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
         a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
+        a.orig = s.orig; a.excl = s.excl;
         hipLaunchKernelGGL(k_energy, dim3(s.ncells), dim3(256), 0, st, a, s.etab, s.pair_ab_e,
                            s.cell_red);
+        if (s.nterms > 0)
+            hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
     }
     hipLaunchKernelGGL(k_sys_reduce, dim3(s.nsys), dim3(256), 0, st, s.geom, s.cell_red, s.atom_ke,
-                       s.red);
+                       s.nterms > 0 ? s.term_e : nullptr, s.term_off, s.red);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;
 }

@@ -87,6 +87,8 @@ class SystemDesc(ctypes.Structure):
         ("particle", ctypes.POINTER(ctypes.c_float)), ("type", ctypes.POINTER(ctypes.c_int32)),
         ("charge", ctypes.POINTER(ctypes.c_float)), ("eps", ctypes.c_float * 8),
         ("sigma", ctypes.c_float * 8), ("mass", ctypes.c_float * 8),
+        ("nterms", ctypes.c_int), ("term_atoms", ctypes.POINTER(ctypes.c_int32)),
+        ("term_params", ctypes.POINTER(ctypes.c_float)), ("excl", ctypes.POINTER(ctypes.c_uint32)),
     ]
 
 
@@ -134,6 +136,8 @@ def lib():
         L.md_rl_system_size.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32), F,
                                         ctypes.POINTER(I)]
         L.md_rl_set_system.argtypes = [P, I, F, ctypes.POINTER(ctypes.c_int32), F]
+        L.md_rl_set_bonded.argtypes = [P, I, I, ctypes.POINTER(ctypes.c_int32), F]
+        L.md_rl_set_exclusions.argtypes = [P, I, ctypes.POINTER(ctypes.c_uint32)]
         L.md_rl_commit.argtypes = [P]
         L.md_rl_get_system.argtypes = [P, I, F]
         L.md_rl_get_system_forces_fixed.argtypes = [P, I, ctypes.POINTER(ctypes.c_int64)]
@@ -270,6 +274,14 @@ class SyntheticSystem:
             self.eps = np.array(d.eps[:self.ntypes], np.float32)
             self.sigma = np.array(d.sigma[:self.ntypes], np.float32)
             self.mass = np.array(d.mass[:self.ntypes], np.float32)
+            self.nterms = int(d.nterms)
+            if self.nterms:
+                self.term_atoms = np.ctypeslib.as_array(d.term_atoms, shape=(self.nterms, 4)).copy()
+                self.term_params = np.ctypeslib.as_array(d.term_params, shape=(self.nterms, 4)).copy()
+            else:
+                self.term_atoms = np.zeros((0, 4), np.int32)
+                self.term_params = np.zeros((0, 4), np.float32)
+            self.excl = np.ctypeslib.as_array(d.excl, shape=(n,)).copy() if d.excl else None
         finally:
             lib().md_rl_free_system(ctypes.byref(d))
 
@@ -305,8 +317,28 @@ class System:
         sysm = cls(s.n, s.box, s.rc, s.eps, s.sigma, s.mass, dt=s.dt, coulomb_k=s.coulomb_k,
                    flags=flags, table=table)
         sysm.set_types(s.type, s.charge if s.coulomb_k else None)
+        if getattr(s, "nterms", 0):
+            sysm.set_bonded(s.term_atoms, s.term_params)
+        if getattr(s, "excl", None) is not None:
+            sysm.set_exclusions(s.excl)
         sysm.set_particles(s.particle)
         return sysm
+
+    def set_bonded(self, atoms, params, sys=0):
+        """Bonded terms (rows of 4 input indices, -1 padded; rows of 4 parameters)."""
+        a = np.ascontiguousarray(atoms, np.int32).reshape(-1, 4)
+        p = _f32(params, (a.shape[0], 4))
+        _check(lib().md_rl_set_bonded(self.handle, sys, a.shape[0],
+                                      a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _fp(p)))
+
+    def set_exclusions(self, mask, sys=0):
+        """Exclusion windows (bit d of mask[i] excludes the pair (i, i + d - 16)); None
+        removes them."""
+        if mask is None:
+            _check(lib().md_rl_set_exclusions(self.handle, sys, None))
+            return
+        m = np.ascontiguousarray(mask, np.uint32)
+        _check(lib().md_rl_set_exclusions(self.handle, sys, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
 
     def set_particles(self, particle):
         a = _f32(particle, (self.n, 11))
@@ -423,6 +455,10 @@ class Batch(System):
                 s0.mass, dt=s0.dt, coulomb_k=s0.coulomb_k, flags=flags, table=table)
         for k, s in enumerate(systems):
             b.set_system(k, s.particle, s.type, s.charge if s.coulomb_k else None)
+            if getattr(s, "nterms", 0):
+                b.set_bonded(s.term_atoms, s.term_params, sys=k)
+            if getattr(s, "excl", None) is not None:
+                b.set_exclusions(s.excl, sys=k)
         b.commit()
         return b
 
