@@ -37,6 +37,7 @@ WORKLOADS = {
     1: "512-atom argon, 34 A box, rc 8.5 A, 4x4x4 cells, dt 2 fs",
     2: "92,224-atom LJ+Coulomb, 108.86x108.86x77.76 A, rc 12 A, 9x9x6 cells, dt 1 fs",
     3: "1,048,576-atom LJ+Coulomb, 219.0 A cube, rc 12 A, 18x18x18 cells, dt 1 fs",
+    5: "128 replicas of 64 chains x 48 united-atom beads, bonded + LJ + Coulomb, dt 2 fs",
 }
 
 
@@ -209,6 +210,96 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     return out
 
 
+def bench_bonded(args):
+    """The bonded workload: 128 replicas of 64 chains x 48 beads, bonded terms, exclusions,
+    range-limited LJ + Coulomb and velocity Verlet, one graph-captured batch."""
+    nrep = args.replicas if args.replicas else 128
+    systems = [mdrl.SyntheticSystem(5, k, seed=args.seed) for k in range(nrep)]
+    if args.dt is not None:
+        for s in systems:
+            s.dt = float(args.dt)
+    s0 = systems[0]
+    b = mdrl.Batch.from_synthetic(systems)
+    relax = args.relax if args.relax else 0
+    if relax:
+        b.minimize(relax, args.relax_step, 1e-4)
+    sched, tune_ms = b.autotune(reps=3)
+    b.step(args.warmup)
+    pe0, ke0, _ = b.energy()
+    b.set_graph(bool(args.graph))
+    t0 = time.perf_counter()
+    b.step_async(args.steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    pairs = int(b.stats().pair_evals)
+    pe1, ke1, _ = b.energy()
+    b.set_graph(False)
+    b.set_timing(True)
+    b.step_async(min(args.steps, 1000))
+    b.sync()
+    st = b.stats()
+    b.close()
+    ms = wall / args.steps * 1e3
+    value = pairs / (ms * 1e-3)
+    # the same replicas as separate contexts, stepped one after another
+    nplain = min(args.replica_steps, args.steps)
+    plain = [mdrl.System.from_synthetic(s) for s in systems]
+    for g in plain:
+        if relax:
+            g.minimize(relax, args.relax_step, 1e-4)
+        g.step(2)
+    t0 = time.perf_counter()
+    for _ in range(nplain):
+        for g in plain:
+            g.step_async(1)
+    for g in plain:
+        g.sync()
+    tp = (time.perf_counter() - t0) / nplain
+    pairs_p = sum(int(g.stats().pair_evals) for g in plain)
+    for g in plain:
+        g.close()
+    natoms = sum(s.n for s in systems)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded chains, seed %d)" % args.seed,
+        "config": {
+            "workload": "%d replicas x 64 chains x 48 beads (%d beads), 0.033 beads/A^3, rc 10 A, "
+                        "bonds/angles/torsions, LJ + Coulomb excluded within 3 bonds, dt %g fs"
+                        % (nrep, natoms, s0.dt),
+            "n_atoms": natoms,
+            "bonded_terms": int(sum(s.nterms for s in systems)),
+            "pair_evals_per_step": pairs,
+            "accumulation": "int64 fixed point (2^-32 kJ/mol/nm)",
+            "pair_eval": "first-order tables, ApoA1 layout",
+            "integrator": "velocity Verlet",
+            "parallelism": "single GPU, one batch",
+        },
+        "ns_per_day": (1e3 / ms) * s0.dt * 0.0864,
+        "force_kernel_ms": st.force_ms,
+        "step_device_ms_eager": st.step_ms,
+        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms),
+        "graph_captured_steps": bool(args.graph),
+        "energy_kj_mol": {"potential_start": pe0, "kinetic_start": ke0, "potential_end": pe1,
+                          "kinetic_end": ke1,
+                          "total_change_rel_ke": (pe1 + ke1 - pe0 - ke0) / ke0},
+        "pre_relaxation": {"iterations": relax, "max_disp_A": args.relax_step},
+        "batched_vs_plain": {"batched_pair_evals_per_s": value, "plain_ms_per_step": tp * 1e3,
+                             "plain_pair_evals_per_s": pairs_p / tp,
+                             "batched_over_plain": value / (pairs_p / tp), "plain_steps": nplain},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,7 +314,8 @@ def main():
     ap.add_argument("--relax-step", type=float, default=0.02, help="max displacement per iteration (A)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--replicas", type=int, default=512, help="batched-vs-plain replicas (0: skip)")
+    ap.add_argument("--replicas", type=int, default=None,
+                    help="replicas of the batched legs (default 512 for workload 2, 128 for 5; 0: skip)")
     ap.add_argument("--replica-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1, help="capture the timed steps in hipGraphs")
     ap.add_argument("--dt", type=float, default=None, help="time step in fs (default: the workload's)")
@@ -237,6 +329,11 @@ def main():
                          "(replicas and spatial domains would shard one process per GPU)\n" % args.gpus)
         sys.exit(2)
 
+    if args.replicas is None:
+        args.replicas = 128 if args.workload == 5 else 512
+    if args.workload == 5:
+        bench_bonded(args)
+        return
     s = mdrl.SyntheticSystem(args.workload, seed=args.seed)
     if args.dt is not None:
         s.dt = float(args.dt)

@@ -165,7 +165,8 @@ int md_rl_set_types(md_rl_ctx *c, const int32_t *type, const float *charge, int 
  *   angle   (theta0 [rad], k [kJ/mol/rad^2]): U = k (theta - theta0)^2  (Bonded/calc_angle_v2.v:113-147)
  *   torsion (k [kJ/mol], n in 1..6, phase [rad], -): U = k (1 + cos(n phi + phase))
  *                                                                     (Bonded/calcTorsion_v2.v:70-274)
- * FP32 forces join the range-limited forces in the same fixed-point buffer (and every
+ * The atoms of one term must lie within 256 consecutive input indices.  FP32 forces
+ * join the range-limited forces in the same fixed-point buffer (and every
  * md_rl_force_eval / step / minimize); their FP64 energy joins the potential energy. */
 int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t *atoms, const float *params);
 /* Nonbonded exclusions of system sys (no counterpart in the reference, whose golden

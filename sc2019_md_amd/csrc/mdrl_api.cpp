@@ -210,7 +210,7 @@ static void free_ctx_device(md_rl_ctx *c)
                     s.type2, s.orig, s.orig2, s.cell, s.perm, s.cell_start, s.cell_next,
                     s.cell_fill, s.fix, s.f32, s.tab, s.etab, s.pair_ab, s.pair_ab_e, s.hk,
                     s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4, s.excl,
-                    s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off};
+                    s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off, s.term_blocks};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
@@ -508,6 +508,36 @@ static int upload_topology(md_rl_ctx *c)
     }
     if (!s.term_off && (rc = dalloc(&s.term_off, (size_t)s.nsys + 1))) return rc;
     s.nterms = (int)atoms.size();
+    // blocks of consecutive terms whose atoms fit one LDS window
+    std::vector<int4> blocks;
+    auto lo_of = [](const int4 &a) {
+        int m = std::min(a.x, a.y);
+        if (a.z >= 0) m = std::min(m, a.z);
+        if (a.w >= 0) m = std::min(m, a.w);
+        return m;
+    };
+    auto hi_of = [](const int4 &a) { return std::max(std::max(a.x, a.y), std::max(a.z, a.w)); };
+    for (size_t t = 0; t < atoms.size();) {
+        int lo = lo_of(atoms[t]), hi = hi_of(atoms[t]);
+        size_t e = t + 1;
+        while (e < atoms.size() && e - t < (size_t)kBondedBlockTerms) {
+            const int nlo = std::min(lo, lo_of(atoms[e])), nhi = std::max(hi, hi_of(atoms[e]));
+            if (nhi - nlo >= kBondedBlockSpan) break;
+            lo = nlo;
+            hi = nhi;
+            ++e;
+        }
+        blocks.push_back(make_int4((int)t, (int)e, lo, hi - lo + 1));
+        t = e;
+    }
+    if (s.term_blocks) { (void)hipFree(s.term_blocks); s.term_blocks = nullptr; }
+    s.nblocks = (int)blocks.size();
+    if (!blocks.empty()) {
+        if ((rc = dalloc(&s.term_blocks, blocks.size()))) return rc;
+        if ((e = hipMemcpyAsync(s.term_blocks, blocks.data(), sizeof(int4) * blocks.size(), hipMemcpyHostToDevice,
+                                c->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync");
+    }
     if (!atoms.empty() &&
         ((e = hipMemcpyAsync(s.term_atoms, atoms.data(), sizeof(int4) * atoms.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
          (e = hipMemcpyAsync(s.term_params, params.data(), sizeof(float4) * params.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess))
@@ -701,6 +731,8 @@ extern "C" int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t
         for (int k = 0; k < na; ++k)
             for (int l = k + 1; l < na; ++l) ok = ok && a[k] != a[l];
         for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(q[k]);
+        for (int k = 0; k < na; ++k)
+            for (int l = 0; l < na; ++l) ok = ok && a[k] - a[l] < kBondedBlockSpan;
         if (na == 4) ok = ok && q[1] >= 1.0f && q[1] <= 6.0f && q[1] == floorf(q[1]);
         if (!ok) {
             set_error("md_rl_set_bonded: term %d of system %d is invalid", t, sys);

@@ -115,6 +115,8 @@ struct DeviceSystem {
     float4 *term_params = nullptr;            // nterms: kernel constants (factor 10 folded in)
     double *term_e = nullptr;                 // nterms: FP64 energy per term (on request)
     int *term_off = nullptr;                  // nsys + 1: terms of each system
+    int4 *term_blocks = nullptr;              // per block: first term, end, first atom, span
+    int nblocks = 0;
     int exp_offset = 121, nseg = 14;
     TableWindow win;
     Schedule sched;
@@ -132,6 +134,8 @@ int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
 int launch_zero_forces(DeviceSystem &s, hipStream_t st);   // forces and pair counters
 int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st);   // red[] per system
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc);
+constexpr int kBondedBlockSpan = 256;     // input indices one block of bonded terms may touch
+constexpr int kBondedBlockTerms = 1024;   // terms per block
 
 // Kernel launchers (mdrl_kernels.hip).
 int launch_eval_pairs(const float *d_tab, int exp_offset, int nseg, const float *ref,

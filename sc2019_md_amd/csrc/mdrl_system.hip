@@ -42,11 +42,11 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 constexpr int TI = 256, QCAP = 128, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
 static_assert(TI <= 256, "queue entries hold 8-bit home indices");
 
-// One neighbour tile: offset o, first atom jt of the tile within its cell, slot of that
-// atom, atoms in the tile and the periodic image shift of the cell.
+// One neighbour tile: offsets [o, oe) of the task's group, each a whole cell (cells that
+// fit the tile are packed together), or a TJ-atom piece starting at atom jt of cell o
+// (the home cell, a cell larger than the tile, or any cell in parity mode); mj atoms.
 struct NbrTile {
-    int o, jt, base, mj;
-    float sx, sy, sz;
+    int o, oe, jt, mj;
 };
 
 struct ForceArgs {
@@ -90,9 +90,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float2 s_ab[64];
     __shared__ int s_task;
     __shared__ unsigned long long s_stat[NWAVE][2];
-    __shared__ int s_io[EXC ? TI : 1];
-    __shared__ uint32_t s_im[EXC ? TI : 1];
+    __shared__ int2 s_iom[EXC ? TI : 1];     // home atoms: input index, exclusion window
     __shared__ int s_jo[EXC ? TJ : 1];
+    __shared__ int2 s_cinfo[14];             // cells of the task: first slot, atom count
+    __shared__ float4 s_cshift[14];          // periodic image shift of each cell
+    __shared__ int s_jslot[2][TJ];           // slot of each neighbour-tile atom (two passes)
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -148,29 +150,48 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
         const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
         const int obeg = a.gstart[grp], oend = a.gstart[grp + 1];
 
+        // the task's cells: first slot, count and image shift, one thread per offset
+        if (tid >= obeg && tid < oend) {
+            const int o = tid;
+            int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
+            float4 sh = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (rx < 0) { rx += nc0; sh.x = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; sh.x = G.box[0]; }
+            if (ry < 0) { ry += nc1; sh.y = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; sh.y = G.box[1]; }
+            if (rz < 0) { rz += nc2; sh.z = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; sh.z = G.box[2]; }
+            const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
+            const int cs = a.cell_start[c];
+            s_cinfo[o] = make_int2(cs, a.cell_start[c + 1] - cs);
+            s_cshift[o] = sh;
+        }
+        lds_barrier();
+
         for (int itile = 0; itile < hcount; itile += TI) {
             const int mi = min(TI, hcount - itile);
-            // The first neighbour tile at or after (offset o, tile jt) of this group that
-            // holds pairs: in the home cell only tiles reaching the diagonal.
+            // The first neighbour tile at or after (offset o, atom jt of its cell) that holds
+            // pairs: in the home cell only pieces reaching the diagonal; cells that fit
+            // are packed into one tile (not in parity mode, where the shift is per tile).
+            // (the counts are read from LDS and made wave-uniform, so the tile loop stays
+            // under scalar control)
+            auto count_of = [&](int o) { return __builtin_amdgcn_readfirstlane(s_cinfo[o].y); };
             auto find = [&](int o, int jt) {
-                NbrTile t{oend, 0, 0, 0, 0.f, 0.f, 0.f};
                 for (; o < oend; ++o, jt = 0) {
-                    int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
-                    t.sx = t.sy = t.sz = 0.f;
-                    if (rx < 0) { rx += nc0; t.sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; t.sx = G.box[0]; }
-                    if (ry < 0) { ry += nc1; t.sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; t.sy = G.box[1]; }
-                    if (rz < 0) { rz += nc2; t.sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; t.sz = G.box[2]; }
-                    const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
-                    const int cstart = a.cell_start[c], ccount = a.cell_start[c + 1] - cstart;
+                    const int cnt = count_of(o);
                     if (o == 0 && jt < itile) jt = max(jt, (itile / TJ) * TJ);   // below the diagonal
-                    if (jt < ccount) {
-                        t.o = o; t.jt = jt; t.base = cstart + jt; t.mj = min(TJ, ccount - jt);
-                        return t;
-                    }
+                    if (jt >= cnt) continue;
+                    if (o == 0 || PAR || cnt > TJ) return NbrTile{o, o + 1, jt, min(TJ, cnt - jt)};
+                    int tot = cnt, oe = o + 1;
+                    while (oe < oend && tot + count_of(oe) <= TJ) tot += count_of(oe++);
+                    return NbrTile{o, oe, 0, tot};
                 }
-                t.o = oend; t.jt = 0; t.base = 0; t.mj = 0;
-                t.sx = t.sy = t.sz = 0.f;
-                return t;
+                return NbrTile{oend, oend, 0, 0};
+            };
+            // slot of atom k of tile t and the shift of its cell
+            auto locate = [&](const NbrTile &t, int k, float4 &sh) {
+                int q = t.o;
+                int kk = k + t.jt;
+                while (q + 1 < t.oe && kk >= s_cinfo[q].y) { kk -= s_cinfo[q].y; ++q; }
+                sh = s_cshift[q];
+                return s_cinfo[q].x + kk;
             };
             // stage the home tile and the first neighbour tile; the neighbour tile of the
             // next pass is loaded into registers before a pass's pairs and written to LDS
@@ -181,33 +202,39 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 s_it[k] = a.type[hstart + itile + k];
                 if (EXC) {
                     const int io = a.orig[hstart + itile + k];
-                    s_io[k] = io;
-                    s_im[k] = a.excl[io];
+                    s_iom[k] = make_int2(io, (int)a.excl[io]);
                 }
             }
             for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
             if (tid < cur.mj) {
-                float4 p = a.pos[cur.base + tid];
-                if (!PAR) { p.x = p.x + cur.sx; p.y = p.y + cur.sy; p.z = p.z + cur.sz; }
+                float4 sh;
+                const int sl = locate(cur, tid, sh);
+                float4 p = a.pos[sl];
+                if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
                 s_j[tid] = p;
-                s_jt[tid] = a.type[cur.base + tid];
-                if (EXC) s_jo[tid] = a.orig[cur.base + tid];
+                s_jt[tid] = a.type[sl];
+                s_jslot[0][tid] = sl;
+                if (EXC) s_jo[tid] = a.orig[sl];
             }
             for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
             lds_barrier();
+            int jpar = 0;
 
             while (cur.o < oend) {
-                const NbrTile nxt = find(cur.o, cur.jt + TJ);
+                const NbrTile nxt = (cur.oe == cur.o + 1) ? find(cur.o, cur.jt + TJ) : find(cur.oe, 0);
                 float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
                 int tf = 0, of = 0;
                 if (tid < nxt.mj) {
-                    pf = a.pos[nxt.base + tid];
-                    tf = a.type[nxt.base + tid];
-                    if (EXC) of = a.orig[nxt.base + tid];
+                    float4 sh;
+                    const int sl = locate(nxt, tid, sh);
+                    pf = a.pos[sl];
+                    tf = a.type[sl];
+                    if (EXC) of = a.orig[sl];
                 }
                 const bool self = (cur.o == 0);
                 const int jtile = cur.jt, mj = cur.mj;
-                const float sx = cur.sx, sy = cur.sy, sz = cur.sz;
+                const float4 csh = s_cshift[cur.o];   // the tile's shift (single-cell tiles)
+                const float sx = csh.x, sy = csh.y, sz = csh.z;
 
                 // queue entry: home index | neighbour index << 8 | type pair << 16
                 auto eval = [&](unsigned e) {
@@ -220,8 +247,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
                                                                a.off, a.nseg, a.win_lo, a.nwin);
                     if (EXC) {
-                        const int d = s_jo[jb] - s_io[ia] + 16;
-                        const bool ex = (unsigned)d < 32u && ((s_im[ia] >> d) & 1u);
+                        const int2 om = s_iom[ia];
+                        const int d = s_jo[jb] - om.x + 16;
+                        const bool ex = (unsigned)d < 32u && (((unsigned)om.y >> d) & 1u);
                         nexcl += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(ex));
                         F = ex ? 0.0f : F;
                     }
@@ -311,30 +339,33 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 // and clear the accumulators for the next pass; the atomics stay in flight
                 const int nclr = 3 * max(mj, nxt.mj);
                 if (F32) {
-                    float *dst = a.f32 + 4 * (size_t)cur.base;
                     for (int k = tid; k < nclr; k += NTHR) {
                         if (k < 3 * mj) {
                             const float v = s_jacc_f[k];
-                            if (v != 0.0f) atomic_add_agent(&dst[4 * (k / 3) + k % 3], v);
+                            if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)s_jslot[jpar][k / 3] + k % 3], v);
                         }
                         s_jacc_f[k] = 0.f;
                     }
                 } else {
-                    long long *dst = a.fix + 3 * (size_t)cur.base;
                     for (int k = tid; k < nclr; k += NTHR) {
                         if (k < 3 * mj) {
                             const unsigned long long v = s_jacc[k];
-                            if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(dst + k), v);
+                            if (v)
+                                atomic_add_agent(reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)s_jslot[jpar][k / 3] + k % 3), v);
                         }
                         s_jacc[k] = 0ull;
                     }
                 }
                 if (tid < nxt.mj) {
-                    if (!PAR) { pf.x = pf.x + nxt.sx; pf.y = pf.y + nxt.sy; pf.z = pf.z + nxt.sz; }
+                    float4 sh;   // slot and shift again from LDS rather than held in registers
+                    const int sl = locate(nxt, tid, sh);
+                    if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
                     s_j[tid] = pf;
                     s_jt[tid] = tf;
+                    s_jslot[jpar ^ 1][tid] = sl;
                     if (EXC) s_jo[tid] = of;
                 }
+                jpar ^= 1;
                 lds_barrier();
                 cur = nxt;
             }
@@ -390,6 +421,8 @@ struct BondedArgs {
     float *f32;
     double *term_e;
     unsigned long long *counters;
+    const int4 *blocks;       // per block: first term, end term, first atom, atom span
+    int nblocks;
 };
 
 __device__ __forceinline__ float img_f(float d, float L)
@@ -443,13 +476,21 @@ __device__ __forceinline__ float angle_atan2f(float y, float x)
 
 // The forces of a term's atoms, added atomically.  In fixed point the last atom takes
 // minus the sum of the others' converted forces, so each term's forces sum to zero exactly.
+// Forces go to a sink: the LDS accumulators of the term block (atom index - first atom of
+// the block), flushed to HBM once per atom by k_bonded.
+struct BondedSink {
+    unsigned long long *acc;   // 3 per atom of the block
+    float *accf;               // FP32-atomics mode: 3 per atom
+    int lo;                    // first input index of the block
+};
+
 template <int NA>
-__device__ __forceinline__ void bonded_add(const BondedArgs &a, const int *slot, const float (*f)[3],
+__device__ __forceinline__ void bonded_add(const BondedSink &k_, const int *atom, const float (*f)[3],
                                            unsigned &flag)
 {
-    if (a.f32) {
+    if (k_.accf) {
         for (int k = 0; k < NA; ++k)
-            for (int d = 0; d < 3; ++d) atomic_add_agent(&a.f32[4 * (size_t)slot[k] + d], f[k][d]);
+            for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.accf[3 * (atom[k] - k_.lo) + d], f[k][d]);
         return;
     }
     unsigned long long sum[3] = {0ull, 0ull, 0ull};
@@ -461,25 +502,22 @@ __device__ __forceinline__ void bonded_add(const BondedArgs &a, const int *slot,
         } else {
             for (int d = 0; d < 3; ++d) q[d] = (long long)(0ull - sum[d]);
         }
-        unsigned long long *dst = reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot[k]);
-        for (int d = 0; d < 3; ++d) atomic_add_agent(&dst[d], (unsigned long long)q[d]);
+        for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.acc[3 * (atom[k] - k_.lo) + d], (unsigned long long)q[d]);
     }
 }
 
-// ENERGY: write the FP64 energy of each term (from the same FP32 intermediates)
-// instead of adding forces.
+// One term.  ENERGY: return the FP64 energy of the term (from the same FP32
+// intermediates) instead of adding forces.
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
+__device__ __forceinline__ double bonded_term(const BondedArgs &a, int t, const BondedSink &sink,
+                                              unsigned &flag)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nterms) return;
     const int4 at = a.atoms[t];
     const float4 pp = a.params[t];
     const int s0 = a.slot_of[at.x], s1 = a.slot_of[at.y];
     const SysGeom &G = a.geom[a.slot_sys[s0]];
     const float4 p1 = a.pos[s0], p2 = a.pos[s1];
     const float x1[3] = {p1.x, p1.y, p1.z}, x2[3] = {p2.x, p2.y, p2.z};
-    unsigned flag = 0;
     double e = 0.0;
     if (at.z < 0) {
         // bond, Bonded/calc_bond_v2.v:83-110
@@ -496,8 +534,8 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
             const float c = -dp / r;
             float f[2][3];
             for (int d = 0; d < 3; ++d) { f[0][d] = c * r12[d]; f[1][d] = -f[0][d]; }
-            const int sl[2] = {s0, s1};
-            bonded_add<2>(a, sl, f, flag);
+            const int ids[2] = {at.x, at.y};
+            bonded_add<2>(sink, ids, f, flag);
         }
     } else if (at.w < 0) {
         // angle, Bonded/calc_angle_v2.v:113-147 (no Urey-Bradley term)
@@ -536,8 +574,8 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
                     f[1][d] = c * g3;
                     f[2][d] = -f[0][d] - f[1][d];
                 }
-                const int sl[3] = {s0, s2, s1};
-                bonded_add<3>(a, sl, f, flag);
+                const int ids[3] = {at.x, at.z, at.y};
+                bonded_add<3>(sink, ids, f, flag);
             }
         }
     } else {
@@ -592,12 +630,46 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
                     f[2][d] = fl;
                     f[3][d] = -(fl + sv);
                 }
-                const int sl[4] = {s0, s1, s3, s2};
-                bonded_add<4>(a, sl, f, flag);
+                const int ids[4] = {at.x, at.y, at.w, at.z};
+                bonded_add<4>(sink, ids, f, flag);
             }
         }
     }
-    if (ENERGY) a.term_e[t] = e;
+    return e;
+}
+
+// FP64 energy of every term (thread per term), for the fixed-order per-system reduction.
+__global__ __launch_bounds__(256) void k_bonded_energy(BondedArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nterms) return;
+    unsigned flag = 0;
+    a.term_e[t] = bonded_term<true>(a, t, BondedSink{nullptr, nullptr, 0}, flag);
+}
+
+// Forces of a block of terms (one workgroup): the terms of a block touch at most
+// kBondedBlockSpan consecutive input indices, so they accumulate in LDS (fixed point, or
+// FP32 in FP32-atomics mode) and reach HBM with one atomic per touched atom and component.
+__global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
+{
+    __shared__ unsigned long long s_acc[3 * kBondedBlockSpan];
+    const int4 blk = a.blocks[blockIdx.x];   // first term, end term, first atom, atom span
+    for (int k = threadIdx.x; k < 3 * kBondedBlockSpan; k += blockDim.x) s_acc[k] = 0ull;
+    __syncthreads();
+    const BondedSink sink{a.f32 ? nullptr : s_acc, a.f32 ? reinterpret_cast<float *>(s_acc) : nullptr, blk.z};
+    unsigned flag = 0;
+    for (int t = blk.x + threadIdx.x; t < blk.y; t += blockDim.x) bonded_term<false>(a, t, sink, flag);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * blk.w; k += blockDim.x) {
+        const int slot = a.slot_of[blk.z + k / 3];
+        if (a.f32) {
+            const float v = reinterpret_cast<const float *>(s_acc)[k];
+            if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)slot + k % 3], v);
+        } else {
+            const unsigned long long v = s_acc[k];
+            if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot + k % 3), v);
+        }
+    }
     if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -610,6 +682,7 @@ static BondedArgs bonded_args(DeviceSystem &s)
     b.fix = (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix;
     b.f32 = (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr;
     b.term_e = s.term_e; b.counters = s.counters;
+    b.blocks = s.term_blocks; b.nblocks = s.nblocks;
     return b;
 }
 
@@ -628,7 +701,7 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
     const bool fast = !(s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC));
     const bool groups_ok = sc.groups == 1 || sc.groups == 2 || sc.groups == 4;
     if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
-    if (s.excl) return fast && sc.apw == 64 && sc.tile == 256;   // exclusions: one variant
+    if (s.excl && !fast) return false;   // exclusions: the int64 table path only
     if (sc.apw == 64 && sc.tile == 256) return true;
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
@@ -639,25 +712,25 @@ static void launch_rl(const ForceArgs &a, dim3 grid, size_t lds, hipStream_t st)
     hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>), grid, dim3(NTHR), lds, st, a);
 }
 
-template <bool COUL>
+template <bool COUL, bool EXC>
 static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, size_t lds, hipStream_t st)
 {
     if (sc.tile == 128) {
-        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 128>(a, grid, lds, st);
-        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 128>(a, grid, lds, st);
-        else launch_rl<COUL, false, false, false, 64, 128>(a, grid, lds, st);
+        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 128, EXC>(a, grid, lds, st);
+        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 128, EXC>(a, grid, lds, st);
+        else launch_rl<COUL, false, false, false, 64, 128, EXC>(a, grid, lds, st);
     } else {
-        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 256>(a, grid, lds, st);
-        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 256>(a, grid, lds, st);
-        else launch_rl<COUL, false, false, false, 64, 256>(a, grid, lds, st);
+        if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 256, EXC>(a, grid, lds, st);
+        else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 256, EXC>(a, grid, lds, st);
+        else launch_rl<COUL, false, false, false, 64, 256, EXC>(a, grid, lds, st);
     }
 }
 
 int launch_force(DeviceSystem &s, hipStream_t st)
 {
     if (s.n == 0) return MDRL_OK;
-    if (s.nterms > 0)
-        hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
+    if (s.nblocks > 0)
+        hipLaunchKernelGGL(k_bonded, dim3(s.nblocks), dim3(256), 0, st, bonded_args(s));
     ForceArgs a;
     a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
     a.geom = s.geom; a.cell_sys = s.cell_sys;
@@ -688,11 +761,11 @@ int launch_force(DeviceSystem &s, hipStream_t st)
         if (coul) launch_rl<true, true, false, false, 64, 256>(a, grid, lds, st);
         else launch_rl<false, true, false, false, 64, 256>(a, grid, lds, st);
     } else if (s.excl) {
-        if (coul) launch_rl<true, false, false, false, 64, 256, true>(a, grid, lds, st);
-        else launch_rl<false, false, false, false, 64, 256, true>(a, grid, lds, st);
+        if (coul) launch_rl_fast<true, true>(a, sc, grid, lds, st);
+        else launch_rl_fast<false, true>(a, sc, grid, lds, st);
     } else {
-        if (coul) launch_rl_fast<true>(a, sc, grid, lds, st);
-        else launch_rl_fast<false>(a, sc, grid, lds, st);
+        if (coul) launch_rl_fast<true, false>(a, sc, grid, lds, st);
+        else launch_rl_fast<false, false>(a, sc, grid, lds, st);
     }
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;
@@ -1126,7 +1199,7 @@ int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
         hipLaunchKernelGGL(k_energy, dim3(s.ncells), dim3(256), 0, st, a, s.etab, s.pair_ab_e,
                            s.cell_red);
         if (s.nterms > 0)
-            hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
+            hipLaunchKernelGGL(k_bonded_energy, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
     }
     hipLaunchKernelGGL(k_sys_reduce, dim3(s.nsys), dim3(256), 0, st, s.geom, s.cell_red, s.atom_ke,
                        s.nterms > 0 ? s.term_e : nullptr, s.term_off, s.red);
