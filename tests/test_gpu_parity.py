@@ -254,3 +254,41 @@ def test_invalid_inputs_raise():
     with pytest.raises(mdrl.MdrlError):
         g.set_particles(bad)
     g.close()
+
+
+def ragged(index=20, width=2.5, seed=7):
+    """A replica (4x4x4 cells) whose atoms with x < 0.3 L are moved into a dense blob:
+    empty cells next to cells of ~1,600 atoms (split into several home and neighbour
+    tiles) and ordinary cells of ~180."""
+    s = mdrl.SyntheticSystem(4, index)
+    L = np.asarray(s.box, np.float64)
+    x = s.particle[:, :3].astype(np.float64)
+    sel = x[:, 0] < 0.3 * L[0]
+    rng = np.random.default_rng(seed)
+    c = L * np.array([0.62, 0.55, 0.5])
+    x[sel] = (c + width * rng.normal(size=(int(sel.sum()), 3))) % L
+    s.particle[:, :3] = x.astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])
+def test_ragged_cells_bit_exact(apoa1, mode):
+    s = ragged()
+    say("ragged cells: n=%d box %.1f mode=%s" % (s.n, s.box[0], mode))
+    flags = {"fast": 0, "parity": mdrl.EVAL_PARITY, "analytic": mdrl.EVAL_ANALYTIC}[mode]
+    omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
+    g = gpu_system(s, flags)
+    st = g.stats()
+    assert st.max_cell_count > 512
+    want, npairs = rl_from_synthetic(s, apoa1, mode=omode).forces_fix()
+    scheds = [(64, 256, 1, 1)]
+    if mode == "fast":
+        scheds = [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256) for gr in (1, 2, 4) for p in (0, 1)]
+    for sc in scheds:
+        g.set_schedule(*sc)
+        g.force_eval()
+        got = g.forces_fixed()
+        assert g.stats().pair_evals == npairs, sc
+        bad = np.nonzero(np.any(got != want, axis=1))[0]
+        assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
+    g.close()
