@@ -376,8 +376,8 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     if (!st) st = dalloc(&s.cell, n);
     if (!st) st = dalloc(&s.perm, n);
     if (!st) st = dalloc(&s.cell_start, (size_t)s.ncells + 1);
-    if (!st) st = dalloc(&s.cell_next, s.ncells);
-    if (!st) st = dalloc(&s.cell_fill, s.ncells);
+    if (!st) st = dalloc(&s.cell_next, (size_t)kCountCopies * s.ncells);
+    if (!st) st = dalloc(&s.cell_fill, (size_t)kCountCopies * s.ncells);
     if (!st) st = (p->flags & MDRL_ACC_FP32) ? dalloc(&s.f32, 4 * (size_t)n) : dalloc(&s.fix, 3 * (size_t)n);
     if (!st) st = dalloc(&s.tab, (size_t)t->nent * 6);
     if (!st) st = dalloc(&s.etab, (size_t)t->nent * 6);
@@ -593,8 +593,8 @@ static int upload(md_rl_ctx *c)
          (e = hipMemcpyAsync(s.type, c->h_type.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
          (e = hipMemcpyAsync(s.orig, orig.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess))
         return hip_fail(e, "hipMemcpyAsync");
-    std::vector<int> zero(s.ncells, 0);
-    if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * s.ncells, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    std::vector<int> zero((size_t)kCountCopies * s.ncells, 0);
+    if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * zero.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync");
     int rc;
     if ((rc = upload_topology(c))) return rc;

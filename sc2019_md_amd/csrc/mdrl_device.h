@@ -91,6 +91,11 @@ __device__ __forceinline__ int run_add(int *count, int key, bool active)
     return hb + (lane - hl);
 }
 
+// Which of the kCountCopies copies of the per-cell counters this block adds into: blocks
+// are dealt round-robin to the XCDs, so each copy is touched from one XCD, and a cell's
+// increments spread over kCountCopies addresses instead of one.
+__device__ __forceinline__ int count_copy() { return (int)(blockIdx.x % kCountCopies); }
+
 template <typename T>
 __device__ __forceinline__ void atomic_add_agent(T *p, T v)
 {

@@ -66,6 +66,9 @@ constexpr int kQueues = 8;
 constexpr int kQueueWord = 8 + 2 * kStatSlots;
 constexpr int kCounterWords = kQueueWord + kQueues;
 
+// Copies of the per-cell counts of the binning (see count_copy in the kernels).
+constexpr int kCountCopies = 8;
+
 constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], bonded PE
 
 // Device state of a batch of systems.
@@ -91,8 +94,8 @@ struct DeviceSystem {
     int *cell = nullptr;                      // cell of each slot after the update
     int *perm = nullptr;                      // new slot -> old slot
     int *cell_start = nullptr;                // ncells + 1
-    int *cell_next = nullptr;                 // ncells: counts of the next binning
-    int *cell_fill = nullptr;                 // ncells
+    int *cell_next = nullptr;                 // kCountCopies x ncells: counts of the next binning
+    int *cell_fill = nullptr;                 // kCountCopies x ncells: fill pointers
     long long *fix = nullptr;                 // n x 3, int64 fixed point
     float *f32 = nullptr;                     // n x 4, FP32 atomics mode
     float *tab = nullptr;                     // force table, 6 floats / entry

@@ -783,7 +783,7 @@ __device__ __forceinline__ int global_cell(const SysGeom &G, float x, float y, f
 }
 
 __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const int *slot_sys,
-                         int *cell, int *count)
+                         int *cell, int *count, int ncells)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 0;
@@ -792,12 +792,13 @@ __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const in
         c = global_cell(geom[slot_sys[s]], p.x, p.y, p.z);
         cell[s] = c;
     }
-    run_add(count, c, s < n);
+    run_add(count + count_copy() * (size_t)ncells, c, s < n);
 }
 
-// pointers; records the largest cell; clears the pair counters and the work queues of
-// pointers; records the largest cell; clears the pair counters and the work counter of
-// the force evaluation that follows.  One workgroup of 1024 threads.
+// Exclusive scan of the next-step counts (kCountCopies copies per cell, summed) into
+// cell_start, and the start of each copy's share of each cell into the fill pointers;
+// clears the counts; records the largest cell; clears the pair counters and the work
+// queues of the force evaluation that follows.  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *start, int *fill,
                                                unsigned long long *counters)
 {
@@ -807,7 +808,12 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
     const int per = (ncells + 1023) / 1024;
     const int b = t * per, e = min(ncells, b + per);
     int sum = 0, mx = 0;
-    for (int c = b; c < e; ++c) { int v = count[c]; sum += v; mx = max(mx, v); }
+    for (int c = b; c < e; ++c) {
+        int v = 0;
+        for (int k = 0; k < kCountCopies; ++k) v += count[(size_t)k * ncells + c];
+        sum += v;
+        mx = max(mx, v);
+    }
     s_sum[t] = sum;
     s_max[t] = mx;
     __syncthreads();
@@ -821,11 +827,13 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
     }
     int run = s_sum[t] - sum;
     for (int c = b; c < e; ++c) {
-        int v = count[c];
         start[c] = run;
-        run += v;
-        count[c] = 0;
-        fill[c] = 0;
+        for (int k = 0; k < kCountCopies; ++k) {
+            const size_t q = (size_t)k * ncells + c;
+            fill[q] = run;
+            run += count[q];
+            count[q] = 0;
+        }
     }
     if (t == 1023) {
         start[ncells] = s_sum[1023];
@@ -834,12 +842,14 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
     if (t < 2 * kStatSlots + kQueues) counters[8 + t] = 0ull;
 }
 
-__global__ void k_scatter(int n, const int *cell, const int *start, int *fill, int *perm)
+// Each slot takes the next place of its cell's share for the copy its block counted into
+// (the order inside a cell is fixed afterwards by k_cell_sort).
+__global__ void k_scatter(int n, const int *cell, int *fill, int *perm, int ncells)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = (s < n) ? cell[s] : 0;
-    const int k = run_add(fill, c, s < n);
-    if (s < n) perm[start[c] + k] = s;
+    const int k = run_add(fill + count_copy() * (size_t)ncells, c, s < n);
+    if (s < n) perm[k] = s;
 }
 
 // Orders each cell's slots by input index.  Cells larger than the LDS capacity keep
@@ -892,7 +902,7 @@ int launch_bin_initial(DeviceSystem &s, hipStream_t st)
 {
     if (s.n > 0)
         hipLaunchKernelGGL(k_assign, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos, s.geom,
-                           s.slot_sys, s.cell, s.cell_next);
+                           s.slot_sys, s.cell, s.cell_next, s.ncells);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;
 }
@@ -903,7 +913,7 @@ int launch_rebin(DeviceSystem &s, hipStream_t st)
                        s.cell_fill, s.counters);
     if (s.n > 0) {
         hipLaunchKernelGGL(k_scatter, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.cell,
-                           s.cell_start, s.cell_fill, s.perm);
+                           s.cell_fill, s.perm, s.ncells);
         hipLaunchKernelGGL(k_cell_sort, dim3(s.ncells), dim3(256), 0, st, s.cell_start, s.perm,
                            s.orig, s.counters);
         hipLaunchKernelGGL(k_gather, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.perm, s.pos,
@@ -946,7 +956,8 @@ __global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel,
                                                 const long long *fix, const float *f32,
                                                 const double *hk, float dt, const SysGeom *geom,
                                                 const int *slot_sys, int first, int kick_drift,
-                                                int *cell, int *count, unsigned long long *counters)
+                                                int *cell, int *count, int ncells,
+                                                unsigned long long *counters)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned flag = 0;
@@ -989,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_update(int n, float4 *pos, float4 *vel,
         c = global_cell(G, xi[0], xi[1], xi[2]);
         cell[s] = c;
     }
-    run_add(count, c, s < n);
+    run_add(count + count_copy() * (size_t)ncells, c, s < n);
     if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1001,7 +1012,7 @@ int launch_update(DeviceSystem &s, float dt, hipStream_t st)
     hipLaunchKernelGGL(k_update, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos, s.vel, s.type,
                        f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.hk, dt, s.geom, s.slot_sys,
                        s.first_step ? 1 : 0, (s.flags & MDRL_KICK_DRIFT) ? 1 : 0, s.cell,
-                       s.cell_next, s.counters);
+                       s.cell_next, s.ncells, s.counters);
     MDRL_HIP(hipGetLastError());
     s.first_step = false;
     return MDRL_OK;
@@ -1016,7 +1027,7 @@ int launch_update(DeviceSystem &s, float dt, hipStream_t st)
 __global__ __launch_bounds__(256) void k_minimize(int n, float4 *pos, const long long *fix,
                                                   const float *f32, double gamma, double dmax,
                                                   const SysGeom *geom, const int *slot_sys,
-                                                  int *cell, int *count,
+                                                  int *cell, int *count, int ncells,
                                                   unsigned long long *counters)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1042,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_minimize(int n, float4 *pos, const long
         c = global_cell(G, xi[0], xi[1], xi[2]);
         cell[s] = c;
     }
-    run_add(count, c, s < n);
+    run_add(count + count_copy() * (size_t)ncells, c, s < n);
     if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1053,7 +1064,7 @@ int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st)
     const bool f32 = s.flags & MDRL_ACC_FP32;
     hipLaunchKernelGGL(k_minimize, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.pos,
                        f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, gamma, dmax, s.geom,
-                       s.slot_sys, s.cell, s.cell_next, s.counters);
+                       s.slot_sys, s.cell, s.cell_next, s.ncells, s.counters);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;
 }
