@@ -221,7 +221,8 @@ int md_rl_get_system(md_rl_ctx *c, int sys, float *particle);
 int md_rl_get_system_forces_fixed(md_rl_ctx *c, int sys, int64_t *fix);
 int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke, double virial[9]);
 
-/* Capture md_rl_step's steps in hipGraphs (two steps per graph launch); off by default. */
+/* Capture md_rl_step's steps in hipGraphs (16 steps per graph launch, then pairs of steps); off
+ * by default. */
 int md_rl_set_graph(md_rl_ctx *c, int on);
 
 /* Force-kernel schedule.  atoms_per_wave: home atoms per 64-lane wavefront (16, 32 or

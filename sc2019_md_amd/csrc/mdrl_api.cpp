@@ -187,6 +187,7 @@ struct md_rl_ctx {
     int timed_steps = 0;
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps, one per buffer parity
+    hipGraphExec_t graph_long[2] = {nullptr, nullptr};   // kGraphLong steps, per parity
     float4 *d_tmp4 = nullptr;         // n: full-step velocities for read-back
     md_rl_table *own_table = nullptr;
     // staged bonded terms (batch-wide input indices, kernel constants) and exclusion
@@ -200,6 +201,8 @@ struct md_rl_ctx {
 static void drop_graphs(md_rl_ctx *c)
 {
     for (auto &g : c->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &g : c->graph_long)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
@@ -868,12 +871,16 @@ static int enqueue_step(md_rl_ctx *c, hipStream_t st, bool events)
 
 // Two steps captured as one graph for the current buffer parity (the rebin swaps the
 // double buffers, so two steps return the pointers to where they started).
-static int capture_pair(md_rl_ctx *c)
+// nsteps (even) steps captured as one graph for the current buffer parity; an even count
+// returns the double buffers to where they started.
+constexpr int kGraphLong = 16;
+static int capture_steps(md_rl_ctx *c, int nsteps, hipGraphExec_t *slot)
 {
     hipGraph_t g = nullptr;
+    const int parity = c->s.parity;
     MDRL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c, c->stream, false);
-    if (!rc) rc = enqueue_step(c, c->stream, false);
+    int rc = MDRL_OK;
+    for (int k = 0; k < nsteps && !rc; ++k) rc = enqueue_step(c, c->stream, false);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -881,7 +888,7 @@ static int capture_pair(md_rl_ctx *c)
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    c->graph[c->s.parity] = ex;
+    slot[parity] = ex;
     return MDRL_OK;
 }
 
@@ -897,8 +904,12 @@ extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
             if ((rc = enqueue_step(c, st, false))) return rc;
             ++k;
         }
+        for (; k + kGraphLong <= nsteps; k += kGraphLong) {
+            if (!c->graph_long[c->s.parity] && (rc = capture_steps(c, kGraphLong, c->graph_long))) return rc;
+            MDRL_HIP(hipGraphLaunch(c->graph_long[c->s.parity], st));
+        }
         for (; k + 2 <= nsteps; k += 2) {
-            if (!c->graph[c->s.parity] && (rc = capture_pair(c))) return rc;
+            if (!c->graph[c->s.parity] && (rc = capture_steps(c, 2, c->graph))) return rc;
             MDRL_HIP(hipGraphLaunch(c->graph[c->s.parity], st));
         }
     }

@@ -104,6 +104,7 @@ def test_graph_steps_bit_identical(which):
     a.step(11)
     a.energy()
     a.step(10)
+    a.step(37)
     want = read(a)
     a.close()
     b = mk()
@@ -111,6 +112,7 @@ def test_graph_steps_bit_identical(which):
     b.step(11)
     b.energy()
     b.step(10)
+    b.step(37)          # two 16-step graphs, two 2-step graphs, one eager step
     got = read(b)
     b.close()
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
