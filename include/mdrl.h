@@ -77,6 +77,31 @@ const char *md_rl_default_table_dir(void);
 /* Loads the shipped ApoA1 tables (offset 121, 14 segments). */
 int md_rl_table_load_default(md_rl_table **out);
 
+/* ---- cell-memory initialisation files ------------------------------------------------ */
+/* The FPGA initialises one position memory per cell from cell_ini_file_X_Y_Z.{mif,hex}
+ * (X, Y, Z from 1; D/CellMemoryModules/{ApoA1,LJArgon}_Mem_Ini_File/,
+ * LJArgon_v_File_64_Cells/), written by M/GenInputCellList_LJArgon.m:140-205 and
+ * M/GenInputCellList_ApoA1.m:138-203: 96-bit words, word 0 = {0, 0, count}, words
+ * 1..count = {z, y, x} big-endian IEEE singles, zero words up to DEPTH.  The format
+ * follows the path's extension: Quartus .mif (as the generator prints it, count in %X)
+ * or Intel HEX (12-byte records, as Quartus converts the .mif).  Flags: MDRL_FILE_CRLF
+ * ends lines with CR LF, as the shipped files do. */
+enum { MDRL_FILE_CRLF = 1 };
+/* Reads one cell file into xyz[cap][3] (x, y, z); returns the particle count or < 0. */
+int md_rl_cell_file_read(const char *path, float *xyz, int cap);
+/* Writes count particles (xyz[count][3]) as a cell file of depth words (depth > count). */
+int md_rl_cell_file_write(const char *path, const float *xyz, int count, int depth, int flags);
+/* The Motion_Update boundary memory cell_ini_boundary_value (M/GenCellBoundary_LJArgon.m:
+ * 48-60): 32-bit words, word k = k * edge.  Read returns the word count. */
+int md_rl_boundary_file_read(const char *path, float *bounds, int cap);
+int md_rl_boundary_file_write(const char *path, float edge, int depth, int flags);
+/* Reads the cell files <dir>/<prefix>X_Y_Z.hex (else .mif) for lo[d] <= X, Y, Z <= hi[d]
+ * (prefix NULL = "cell_ini_file_") in the generator's order, X outermost, into particle
+ * rows [cap][11] (positions in cols 0-2, the rest zero; md_rl_set_particles' layout) and,
+ * if cell is not NULL, each particle's (X, Y, Z) into cell[cap][3]; returns the total. */
+int md_rl_cell_dir_read(const char *dir, const char *prefix, const int lo[3], const int hi[3],
+                        float *particle, int32_t *cell, int cap);
+
 /* ---- pair batch ---------------------------------------------------------------- */
 /* Evaluates independent (reference, neighbour) pairs.  Replaces the OpenCL kernel
  * LJ(ref_id, neighbor_id, ref_x..z, neighbor_x..z, Force_out_x..z)
@@ -175,6 +200,34 @@ int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t *atoms, co
  * energies and pair counts.  Windows must be symmetric with bit 16 clear; NULL removes
  * them.  Needs the default int64 table path. */
 int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask);
+/* Summation hook for forces computed outside this library (a long-range solver, an
+ * applied field, restraints).  The reference sums short-range, long-range and bonded
+ * forces per particle before Motion_Update (Summation_Logic/Summation_Logic.v:3-6,
+ * Force_Summation_Unit.v:62-126), its long-range unit taking {x, y, z, q} records and
+ * returning force records (Long_Range/Top_Electro.sv:3-20).  After the range-limited and
+ * bonded forces of every force evaluation, step and minimisation, the library writes the
+ * records of the current positions in input order, clears force and energy, calls
+ * fn(user, view, stream) and adds view->force into the accumulators: each component is
+ * converted like a pair force, rint(f * 10 * 2^32) in int64 (FP32 mode: f * 10 added in
+ * FP32).  md_rl_energy / md_rl_system_energy call fn with want_energy = 1, add
+ * view->energy to the potential energy and discard the forces.
+ * fn must enqueue its work on the stream it is given (a hipStream_t as void*) and not
+ * synchronise the device: with md_rl_set_graph on, the calls made while a step is
+ * captured are replayed by every later launch of that graph, so the work must depend on
+ * the view's buffers only (their addresses stay fixed).  A nonzero return aborts the
+ * calling entry point with MDRL_EINVAL.  fn = NULL removes the hook. */
+typedef struct md_rl_lr_view {
+    int nsys;                  /* systems in the context */
+    int n;                     /* atoms over all systems */
+    const int32_t *atom_off;   /* host, nsys + 1: system k holds input indices [atom_off[k], atom_off[k+1]) */
+    const float *box;          /* host, nsys x 3 (A) */
+    const float *xq;           /* device, n x 4: x, y, z (A, in [0, box)), charge (e); input order */
+    float *force;              /* device, n x 3: the provider's forces (kJ/mol/A), input order; zero on entry */
+    double *energy;            /* device, nsys: the provider's energy per system (kJ/mol); zero on entry */
+    int want_energy;           /* 1 when called for md_rl_energy / md_rl_system_energy */
+} md_rl_lr_view;
+typedef int (*md_rl_lr_fn)(void *user, const md_rl_lr_view *view, void *hip_stream);
+int md_rl_set_long_range(md_rl_ctx *c, md_rl_lr_fn fn, void *user);
 /* Forces at the current positions (cols 6-8 after md_rl_get_particles). */
 int md_rl_force_eval(md_rl_ctx *c);
 /* nsteps velocity-Verlet (or kick-drift) steps: force, Motion_Update, rebin. */

@@ -286,6 +286,52 @@ def motion_update(pos4, vel3, fixf, types, hk, mass, dt, box, first, kick_drift=
     return pos4, vel3, ke.value
 
 
+# ---------------------------------------------------------------------------- summation hook
+def lr_fix(f):
+    """Forces of an outside provider (kJ/mol/A, FP32) in the fixed-point unit of the
+    accumulators: rint(f * 10 * 2^32), ties to even, saturated at +-2^62, NaN to 0.  The
+    reference adds its short-range, long-range and bonded forces in FP32 per particle
+    (Summation_Logic/Force_Summation_Unit.v:62-126, long-range + short-range first, then
+    bonded); here the three meet in int64, where the sum does not depend on the order."""
+    x = np.asarray(f, np.float32).astype(np.float64) * 42949672960.0
+    x = np.where(np.isnan(x), 0.0, np.clip(x, -2.0 ** 62, 2.0 ** 62))
+    return np.rint(x).astype(np.int64)
+
+
+# ---------------------------------------------------------------------------- cell init files
+def generator_positions(raw_xyz):
+    """The cell-file generators' preprocessing (M/GenInputCellList_LJArgon.m:86-99,
+    M/GenInputCellList_ApoA1.m:84-97): raw positions minus the per-axis minimum, in FP64,
+    stored as FP32."""
+    raw = np.asarray(raw_xyz, np.float64).reshape(-1, 3)
+    return (raw - raw.min(axis=0)).astype(np.float32)
+
+
+def generator_cell_lists(pos, edge, ncell, rule="floor"):
+    """The particle-to-cell loop of M/GenInputCellList_LJArgon.m:105-135 (rule "floor":
+    cell = floor(x / edge) + 1) and M/GenInputCellList_ApoA1.m:103-133 (rule "ceil":
+    ceil(x / edge)), per coordinate, FP32 division, a coordinate of exactly 0 in cell 1;
+    particles outside 1..ncell are dropped.  Returns ({(X, Y, Z): k x 3 FP32 positions in
+    input order}, number dropped)."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    e = np.float32(edge)
+    lists, dropped = {}, 0
+    for p in pos:
+        c = []
+        for d in range(3):
+            if p[d] == 0:
+                c.append(1)
+            elif rule == "floor":
+                c.append(int(math.floor(float(np.float32(p[d] / e)))) + 1)
+            else:
+                c.append(int(math.ceil(float(np.float32(p[d] / e)))))
+        if all(1 <= c[d] <= ncell[d] for d in range(3)):
+            lists.setdefault(tuple(c), []).append(p)
+        else:
+            dropped += 1
+    return {k: np.array(v, np.float32) for k, v in lists.items()}, dropped
+
+
 def run_reference_cmodel(pdb_text, workdir):
     """Runs the reference C model (built under _ref/ from the reference sources) on a PDB.
 

@@ -196,6 +196,16 @@ struct md_rl_ctx {
     std::vector<std::vector<float4>> t_params;
     std::vector<std::vector<uint32_t>> t_excl;
     size_t cap_terms = 0;
+    // summation hook: provider, its device records / forces / energies, plain charges in
+    // input order, and the host geometry handed to it
+    md_rl_lr_fn lr_fn = nullptr;
+    void *lr_user = nullptr;
+    float4 *d_lr_xq = nullptr;        // n
+    float *d_lr_f = nullptr;          // n x 3
+    double *d_lr_e = nullptr;         // nsys
+    float *d_q = nullptr;             // n
+    std::vector<int32_t> lr_off;      // nsys + 1
+    std::vector<float> lr_box;        // nsys x 3
 };
 
 static void drop_graphs(md_rl_ctx *c)
@@ -213,7 +223,8 @@ static void free_ctx_device(md_rl_ctx *c)
                     s.type2, s.orig, s.orig2, s.cell, s.perm, s.cell_start, s.cell_next,
                     s.cell_fill, s.fix, s.f32, s.tab, s.etab, s.pair_ab, s.pair_ab_e, s.hk,
                     s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4, s.excl,
-                    s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off, s.term_blocks};
+                    s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off, s.term_blocks,
+                    c->d_lr_xq, c->d_lr_f, c->d_lr_e, c->d_q};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
@@ -565,6 +576,36 @@ static int upload_topology(md_rl_ctx *c)
     return MDRL_OK;
 }
 
+// The summation hook after the range-limited and bonded forces of one evaluation:
+// records out in input order, the provider, its forces into the accumulators (or, for an
+// energy, its per-system energies left in d_lr_e).
+static int run_long_range(md_rl_ctx *c, hipStream_t st, bool energy)
+{
+    if (!c->lr_fn) return MDRL_OK;
+    DeviceSystem &s = c->s;
+    int rc = launch_lr_export(s, c->d_q, c->d_lr_xq, c->d_lr_f, c->d_lr_e, st);
+    if (rc) return rc;
+    md_rl_lr_view v;
+    v.nsys = s.nsys;
+    v.n = s.n;
+    v.atom_off = c->lr_off.data();
+    v.box = c->lr_box.data();
+    v.xq = reinterpret_cast<const float *>(c->d_lr_xq);
+    v.force = c->d_lr_f;
+    v.energy = c->d_lr_e;
+    v.want_energy = energy ? 1 : 0;
+    const int r = c->lr_fn(c->lr_user, &v, (void *)st);
+    if (r) { set_error("long-range provider returned %d", r); return MDRL_EINVAL; }
+    return energy ? MDRL_OK : launch_lr_import(s, c->d_lr_f, st);
+}
+
+// Range-limited and bonded forces, then the summation hook.
+static int forces(md_rl_ctx *c, hipStream_t st)
+{
+    int rc = launch_force(c->s, st);
+    return rc ? rc : run_long_range(c, st, false);
+}
+
 // Uploads the staged input of every system, bins it and computes the forces.
 static int upload(md_rl_ctx *c)
 {
@@ -599,12 +640,15 @@ static int upload(md_rl_ctx *c)
     std::vector<int> zero((size_t)kCountCopies * s.ncells, 0);
     if ((e = hipMemcpyAsync(s.cell_next, zero.data(), sizeof(int) * zero.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync");
+    if (c->d_q && n > 0 &&
+        (e = hipMemcpyAsync(c->d_q, c->h_q.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
     int rc;
     if ((rc = upload_topology(c))) return rc;
     if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream))) return rc;
     s.first_step = true;
     drop_graphs(c);
-    if ((rc = launch_zero_forces(s, c->stream)) || (rc = launch_force(s, c->stream))) return rc;
+    if ((rc = launch_zero_forces(s, c->stream)) || (rc = forces(c, c->stream))) return rc;
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     c->have_particles = true;
     c->dirty = false;
@@ -790,6 +834,33 @@ extern "C" int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask)
     return (c->s.nsys == 1 && c->have_particles) ? upload(c) : MDRL_OK;
 }
 
+extern "C" int md_rl_set_long_range(md_rl_ctx *c, md_rl_lr_fn fn, void *user)
+{
+    if (!c) { set_error("md_rl_set_long_range: bad argument"); return MDRL_EINVAL; }
+    DeviceSystem &s = c->s;
+    if (fn && !c->d_lr_xq) {
+        int st = dalloc(&c->d_lr_xq, s.n);
+        if (!st) st = dalloc(&c->d_lr_f, 3 * (size_t)s.n);
+        if (!st) st = dalloc(&c->d_lr_e, s.nsys);
+        if (!st) st = dalloc(&c->d_q, s.n);
+        if (st) return st;
+        c->lr_off.assign(s.nsys + 1, 0);
+        c->lr_box.assign(3 * (size_t)s.nsys, 0.0f);
+        for (int r = 0; r < s.nsys; ++r) {
+            c->lr_off[r] = s.h_geom[r].atom_off;
+            for (int d = 0; d < 3; ++d) c->lr_box[3 * r + d] = s.h_geom[r].box[d];
+        }
+        c->lr_off[s.nsys] = s.n;
+        if (s.n > 0) MDRL_HIP(hipMemcpy(c->d_q, c->h_q.data(), sizeof(float) * s.n, hipMemcpyHostToDevice));
+    }
+    c->lr_fn = fn;
+    c->lr_user = user;
+    drop_graphs(c);
+    // forces already held were summed without (or with) the old hook: evaluate again
+    if (c->have_particles && !c->dirty) return md_rl_force_eval(c);
+    return MDRL_OK;
+}
+
 extern "C" int md_rl_commit(md_rl_ctx *c)
 {
     if (!c) return MDRL_EINVAL;
@@ -846,7 +917,9 @@ extern "C" int md_rl_force_eval(md_rl_ctx *c)
     if ((rc = launch_zero_forces(c->s, c->stream))) return rc;
     if (ev) (void)hipEventRecord(ev->f0, c->stream);
     if ((rc = launch_force(c->s, c->stream))) return rc;
-    if (ev) { (void)hipEventRecord(ev->f1, c->stream); (void)hipEventRecord(ev->s1, c->stream); }
+    if (ev) (void)hipEventRecord(ev->f1, c->stream);
+    if ((rc = run_long_range(c, c->stream, false))) return rc;
+    if (ev) (void)hipEventRecord(ev->s1, c->stream);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     collect_times(c);
@@ -865,7 +938,9 @@ static int enqueue_step(md_rl_ctx *c, hipStream_t st, bool events)
     if ((rc = launch_rebin(s, st))) return rc;
     if (ev) (void)hipEventRecord(ev->f0, st);
     if ((rc = launch_force(s, st))) return rc;
-    if (ev) { (void)hipEventRecord(ev->f1, st); (void)hipEventRecord(ev->s1, st); }
+    if (ev) (void)hipEventRecord(ev->f1, st);
+    if ((rc = run_long_range(c, st, false))) return rc;
+    if (ev) (void)hipEventRecord(ev->s1, st);
     return MDRL_OK;
 }
 
@@ -945,7 +1020,7 @@ extern "C" int md_rl_minimize(md_rl_ctx *c, int iterations, float max_disp, floa
     DeviceSystem &s = c->s;
     for (int k = 0; k < iterations; ++k) {
         if ((rc = launch_minimize(s, gamma, max_disp, c->stream)) || (rc = launch_rebin(s, c->stream)) ||
-            (rc = launch_force(s, c->stream)))
+            (rc = forces(c, c->stream)))
             return rc;
     }
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -1028,14 +1103,19 @@ extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)
     return md_rl_get_system_forces_fixed(c, 0, fix);
 }
 
-static int energies(md_rl_ctx *c, std::vector<double> &red)
+// Per-system FP64 reductions (kRedStride each) and the summation hook's energies.
+static int energies(md_rl_ctx *c, std::vector<double> &red, std::vector<double> &lr)
 {
     int rc = ensure_uploaded(c);
     if (rc) return rc;
     DeviceSystem &s = c->s;
     if ((rc = launch_energy(s, nullptr, c->stream))) return rc;
+    if ((rc = run_long_range(c, c->stream, true))) return rc;
     red.resize((size_t)kRedStride * s.nsys);
+    lr.assign(s.nsys, 0.0);
     hipError_t e = hipMemcpyAsync(red.data(), s.red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && c->lr_fn && s.nsys > 0)
+        e = hipMemcpyAsync(lr.data(), c->d_lr_e, sizeof(double) * s.nsys, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e == hipSuccess ? MDRL_OK : hip_fail(e, "hipMemcpy");
 }
@@ -1043,12 +1123,12 @@ static int energies(md_rl_ctx *c, std::vector<double> &red)
 extern "C" int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke, double virial[9])
 {
     if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_system_energy: bad argument"); return MDRL_EINVAL; }
-    std::vector<double> red;
-    int rc = energies(c, red);
+    std::vector<double> red, lr;
+    int rc = energies(c, red, lr);
     if (rc) return rc;
     const double *r = red.data() + (size_t)kRedStride * sys;
     if (ke) *ke = r[0];
-    if (pe) *pe = r[1] + r[11];
+    if (pe) *pe = r[1] + r[11] + lr[sys];
     if (virial) for (int k = 0; k < 9; ++k) virial[k] = r[2 + k];
     return MDRL_OK;
 }
@@ -1056,14 +1136,16 @@ extern "C" int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke
 extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9])
 {
     if (!c) { set_error("md_rl_energy: bad argument"); return MDRL_EINVAL; }
-    std::vector<double> red;
-    int rc = energies(c, red);
+    std::vector<double> red, lr;
+    int rc = energies(c, red, lr);
     if (rc) return rc;
-    double t[12] = {0};
-    for (int r = 0; r < c->s.nsys; ++r)
+    double t[12] = {0}, tlr = 0.0;
+    for (int r = 0; r < c->s.nsys; ++r) {
         for (int k = 0; k < 12; ++k) t[k] += red[(size_t)kRedStride * r + k];
+        tlr += lr[r];
+    }
     if (ke) *ke = t[0];
-    if (pe) *pe = t[1] + t[11];
+    if (pe) *pe = t[1] + t[11] + tlr;
     if (virial) for (int k = 0; k < 9; ++k) virial[k] = t[2 + k];
     return MDRL_OK;
 }

@@ -137,6 +137,10 @@ int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
 int launch_zero_forces(DeviceSystem &s, hipStream_t st);   // forces and pair counters
 int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st);   // red[] per system
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc);
+// Summation hook: {x, y, z, q} records in input order out of the slots (clearing the
+// provider's forces f, n x 3, and energies e, nsys), and the provider's forces back in.
+int launch_lr_export(DeviceSystem &s, const float *q, float4 *xq, float *f, double *e, hipStream_t st);
+int launch_lr_import(DeviceSystem &s, const float *f, hipStream_t st);
 constexpr int kBondedBlockSpan = 256;     // input indices one block of bonded terms may touch
 constexpr int kBondedBlockTerms = 1024;   // terms per block
 

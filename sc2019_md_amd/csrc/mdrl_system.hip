@@ -1240,4 +1240,85 @@ int launch_zero_forces(DeviceSystem &s, hipStream_t st)
     return MDRL_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// summation hook: forces from a provider outside the range-limited path
+// ---------------------------------------------------------------------------------
+// The reference adds short-range, long-range and bonded forces per particle before
+// Motion_Update (Summation_Logic/Summation_Logic.v:3-6, Force_Summation_Unit.v:62-126);
+// its long-range unit exchanges {x, y, z, q} records for force records
+// (Long_Range/Top_Electro.sv:3-20).  Here the provider sees input order, so it needs no
+// knowledge of the cell sort: k_lr_export writes the records and clears the provider's
+// outputs, k_lr_import adds the provider's forces into the slot accumulators.
+
+// rint(f * 10 * 2^32) as int64, saturated at +-2^62: kJ/mol/A to the fixed-point unit.
+// The product is exact in FP64 (24-bit mantissa times 5 * 2^33), so one rounding.
+__device__ __forceinline__ long long fix_of_kj_per_a(float f, unsigned &flag)
+{
+    const double x = (double)f * 42949672960.0;
+    if (fabs(x) < 0x1p51) return __double_as_longlong(x + 6755399441055744.0) - 0x4338000000000000LL;
+    if (x != x) { flag |= 2u; return 0; }
+    if (x > 0x1p62) { flag |= 1u; return 1LL << 62; }
+    if (x < -0x1p62) { flag |= 1u; return -(1LL << 62); }
+    return (long long)rint(x);
+}
+
+__global__ __launch_bounds__(256) void k_lr_export(int n, int nsys, const float4 *__restrict__ pos,
+                                                   const int *__restrict__ orig, const float *__restrict__ q,
+                                                   float4 *__restrict__ xq, float *__restrict__ f,
+                                                   double *__restrict__ e)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys) e[k] = 0.0;
+    if (k >= n) return;
+    const int i = orig[k];
+    const float4 p = pos[k];
+    xq[i] = make_float4(p.x, p.y, p.z, q[i]);
+    f[3 * (size_t)i] = 0.0f;
+    f[3 * (size_t)i + 1] = 0.0f;
+    f[3 * (size_t)i + 2] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_lr_import(int n, const int *__restrict__ orig, const float *__restrict__ f,
+                                                   long long *__restrict__ fix, float *__restrict__ f32,
+                                                   unsigned long long *counters)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = orig[k];
+    const float fx = f[3 * (size_t)i], fy = f[3 * (size_t)i + 1], fz = f[3 * (size_t)i + 2];
+    if (f32) {
+        float4 a = reinterpret_cast<float4 *>(f32)[k];
+        a.x += fx * 10.0f;
+        a.y += fy * 10.0f;
+        a.z += fz * 10.0f;
+        reinterpret_cast<float4 *>(f32)[k] = a;
+        return;
+    }
+    unsigned flag = 0;
+    fix[3 * (size_t)k] += fix_of_kj_per_a(fx, flag);
+    fix[3 * (size_t)k + 1] += fix_of_kj_per_a(fy, flag);
+    fix[3 * (size_t)k + 2] += fix_of_kj_per_a(fz, flag);
+    if (flag) __hip_atomic_fetch_or(&counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int launch_lr_export(DeviceSystem &s, const float *q, float4 *xq, float *f, double *e, hipStream_t st)
+{
+    const int m = std::max(s.n, s.nsys);
+    if (m == 0) return MDRL_OK;
+    hipLaunchKernelGGL(k_lr_export, dim3(nblk(m, 256)), dim3(256), 0, st, s.n, s.nsys, s.pos, s.orig, q, xq, f, e);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
+int launch_lr_import(DeviceSystem &s, const float *f, hipStream_t st)
+{
+    if (s.n == 0) return MDRL_OK;
+    const bool f32 = s.flags & MDRL_ACC_FP32;
+    hipLaunchKernelGGL(k_lr_import, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.orig, f,
+                       f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.counters);
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
 }  // namespace mdrl

@@ -92,6 +92,19 @@ class SystemDesc(ctypes.Structure):
     ]
 
 
+class LRView(ctypes.Structure):
+    """md_rl_lr_view: what a summation-hook provider is handed (device addresses as ints)."""
+    _fields_ = [
+        ("nsys", ctypes.c_int), ("n", ctypes.c_int),
+        ("atom_off", ctypes.POINTER(ctypes.c_int32)), ("box", ctypes.POINTER(ctypes.c_float)),
+        ("xq", ctypes.c_void_p), ("force", ctypes.c_void_p), ("energy", ctypes.c_void_p),
+        ("want_energy", ctypes.c_int),
+    ]
+
+
+LR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(LRView), ctypes.c_void_p)
+
+
 def lib():
     """Loads libmdrl.so; raises if it has not been built."""
     global _lib
@@ -154,6 +167,13 @@ def lib():
         L.md_rl_make_system.argtypes = [ctypes.POINTER(SystemDesc), I, I, ctypes.c_uint64]
         L.md_rl_free_system.argtypes = [ctypes.POINTER(SystemDesc)]
         L.md_rl_free_system.restype = None
+        L.md_rl_set_long_range.argtypes = [P, LR_FN, P]
+        L.md_rl_cell_file_read.argtypes = [ctypes.c_char_p, F, I]
+        L.md_rl_cell_file_write.argtypes = [ctypes.c_char_p, F, I, I, I]
+        L.md_rl_boundary_file_read.argtypes = [ctypes.c_char_p, F, I]
+        L.md_rl_boundary_file_write.argtypes = [ctypes.c_char_p, ctypes.c_float, I, I]
+        L.md_rl_cell_dir_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), F, ctypes.POINTER(ctypes.c_int32), I]
         _lib = L
     return _lib
 
@@ -340,6 +360,31 @@ class System:
         m = np.ascontiguousarray(mask, np.uint32)
         _check(lib().md_rl_set_exclusions(self.handle, sys, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
 
+    def set_long_range(self, fn):
+        """Summation hook (md_rl_set_long_range): fn(view, stream) -> int is called with an
+        LRView after the range-limited and bonded forces of every evaluation and adds
+        forces into view.force on that stream (see host_provider for a numpy one); None
+        removes it."""
+        old = getattr(self, "_lr", None)
+        if fn is None:
+            _check(lib().md_rl_set_long_range(self.handle, LR_FN(), None))
+            self._lr = self._lr_old = None
+            return
+
+        def trampoline(user, view, stream):
+            try:
+                return int(fn(view.contents, stream) or 0)
+            except Exception:
+                import traceback
+                traceback.print_exc()
+                return -1
+
+        # the library keeps the hook even when its first evaluation fails, so the callback
+        # object stays referenced from here until it is replaced
+        self._lr_old = old
+        self._lr = LR_FN(trampoline)
+        _check(lib().md_rl_set_long_range(self.handle, self._lr, None))
+
     def set_particles(self, particle):
         a = _f32(particle, (self.n, 11))
         _check(lib().md_rl_set_particles(self.handle, _fp(a), self.n))
@@ -489,6 +534,190 @@ class Batch(System):
         vir = (ctypes.c_double * 9)()
         _check(lib().md_rl_system_energy(self.handle, sys, ctypes.byref(pe), ctypes.byref(ke), vir))
         return pe.value, ke.value, np.array(vir[:]).reshape(3, 3)
+
+
+# ------------------------------------------------------------ cell-memory initialisation files
+FILE_CRLF = 1
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def _count(code):
+    if code < 0:
+        raise MdrlError(code, lib().md_rl_last_error().decode())
+    return code
+
+
+def read_cell_file(path, cap=4096):
+    """One cell_ini_file_X_Y_Z.{mif,hex}: the particles' x, y, z (count x 3, float32)."""
+    xyz = np.zeros((cap, 3), np.float32)
+    n = _count(lib().md_rl_cell_file_read(_path(path), _fp(xyz), cap))
+    return xyz[:n].copy()
+
+
+def write_cell_file(path, xyz, depth, crlf=True):
+    """Writes particles (count x 3: x, y, z) as a cell file of depth words (.mif or .hex
+    by extension), as the reference's generator and Quartus' conversion write it."""
+    a = _f32(xyz, (-1, 3))
+    _check(lib().md_rl_cell_file_write(_path(path), _fp(a), a.shape[0], depth, FILE_CRLF if crlf else 0))
+
+
+def read_boundary_file(path, cap=4096):
+    b = np.zeros(cap, np.float32)
+    n = _count(lib().md_rl_boundary_file_read(_path(path), _fp(b), cap))
+    return b[:n].copy()
+
+
+def write_boundary_file(path, edge, depth, crlf=True):
+    _check(lib().md_rl_boundary_file_write(_path(path), edge, depth, FILE_CRLF if crlf else 0))
+
+
+def read_cell_dir(directory, lo, hi, prefix=None, cap=1 << 22):
+    """Every cell file <prefix>X_Y_Z of a directory for lo <= (X, Y, Z) <= hi (from 1):
+    particle rows (n x 11, positions in cols 0-2) in the generator's order and each
+    particle's (X, Y, Z)."""
+    lo_ = (ctypes.c_int * 3)(*lo)
+    hi_ = (ctypes.c_int * 3)(*hi)
+    part = np.zeros((cap, 11), np.float32)
+    cell = np.zeros((cap, 3), np.int32)
+    n = _count(lib().md_rl_cell_dir_read(_path(directory), None if prefix is None else prefix.encode(),
+                                         lo_, hi_, _fp(part), cell.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         cap))
+    return part[:n].copy(), cell[:n].copy()
+
+
+def bin_cells(xyz, edge, rule="floor"):
+    """Cells (from 1) of the reference's generators: floor(x / edge) + 1
+    (M/GenInputCellList_LJArgon.m:101-136) or ceil(x / edge) (M/GenInputCellList_ApoA1.m:
+    99-134), FP32 division, and cell 1 for a coordinate of exactly 0."""
+    x = _f32(xyz, (-1, 3))
+    q = x / np.float32(edge)
+    c = (np.floor(q) + 1) if rule == "floor" else np.ceil(q)
+    return np.where(x != 0, c, 1).astype(np.int64)
+
+
+def write_cell_dir(directory, xyz, edge, lo, hi, depth, rule="floor", ext=".hex", crlf=True,
+                   prefix="cell_ini_file_"):
+    """Bins particles (n x 3, already in the cell frame) as the reference's generators do and
+    writes one cell file per cell lo <= (X, Y, Z) <= hi, particles in input order; returns
+    the number of particles that fell in no written cell."""
+    x = _f32(xyz, (-1, 3))
+    cells = bin_cells(x, edge, rule)
+    written = 0
+    for cx in range(lo[0], hi[0] + 1):
+        for cy in range(lo[1], hi[1] + 1):
+            for cz in range(lo[2], hi[2] + 1):
+                sel = np.all(cells == (cx, cy, cz), axis=1)
+                write_cell_file(os.path.join(directory, "%s%d_%d_%d%s" % (prefix, cx, cy, cz, ext)),
+                                x[sel], depth, crlf)
+                written += int(sel.sum())
+    return x.shape[0] - written
+
+
+# ------------------------------------------------------------------ summation-hook helpers
+_hip = None
+
+
+def hip():
+    """The HIP runtime (libamdhip64) through ctypes, for providers written in Python."""
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                H = ctypes.CDLL(name)
+                break
+            except OSError:
+                H = None
+        if H is None:
+            raise ImportError("libamdhip64.so not found")
+        P, Z = ctypes.c_void_p, ctypes.c_size_t
+        H.hipMalloc.argtypes = [ctypes.POINTER(P), Z]
+        H.hipFree.argtypes = [P]
+        H.hipMemcpy.argtypes = [P, P, Z, ctypes.c_int]
+        H.hipMemcpyAsync.argtypes = [P, P, Z, ctypes.c_int, P]
+        H.hipStreamSynchronize.argtypes = [P]
+        H.hipStreamIsCapturing.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        _hip = H
+    return _hip
+
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+def _hipcheck(e, what):
+    if e != 0:
+        raise RuntimeError("%s failed: HIP error %d" % (what, e))
+
+
+class DeviceBuffer:
+    """hipMalloc'd bytes holding a numpy array (for device-only providers)."""
+
+    def __init__(self, array):
+        a = np.ascontiguousarray(array)
+        self.nbytes = a.nbytes
+        self.ptr = ctypes.c_void_p()
+        _hipcheck(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)), "hipMalloc")
+        _hipcheck(hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes, H2D), "hipMemcpy")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def host_provider(fn):
+    """Wraps fn(xq, box, atom_off, want_energy) -> (force, energy) computed in numpy
+    (xq: n x 4 float32 {x, y, z, q} in input order; box: nsys x 3; atom_off: nsys + 1;
+    force: n x 3 kJ/mol/A; energy: nsys kJ/mol or None) as a summation-hook provider.  It
+    waits for the stream and copies through host memory, so it cannot run inside a
+    captured graph: with graphs on it returns an error."""
+
+    def provider(view, stream):
+        H = hip()
+        cap = ctypes.c_int(0)
+        _hipcheck(H.hipStreamIsCapturing(stream, ctypes.byref(cap)), "hipStreamIsCapturing")
+        if cap.value:
+            return -2
+        n, nsys = view.n, view.nsys
+        _hipcheck(H.hipStreamSynchronize(stream), "hipStreamSynchronize")
+        xq = np.zeros((n, 4), np.float32)
+        if n:
+            _hipcheck(H.hipMemcpy(xq.ctypes.data_as(ctypes.c_void_p), view.xq, xq.nbytes, D2H), "hipMemcpy")
+        box = np.ctypeslib.as_array(view.box, (nsys, 3)).copy()
+        off = np.ctypeslib.as_array(view.atom_off, (nsys + 1,)).copy()
+        force, energy = fn(xq, box, off, bool(view.want_energy))
+        f = np.ascontiguousarray(force, np.float32).reshape(n, 3)
+        if n:
+            _hipcheck(H.hipMemcpy(view.force, f.ctypes.data_as(ctypes.c_void_p), f.nbytes, H2D), "hipMemcpy")
+        if energy is not None:
+            e = np.ascontiguousarray(energy, np.float64).reshape(nsys)
+            _hipcheck(H.hipMemcpy(view.energy, e.ctypes.data_as(ctypes.c_void_p), e.nbytes, H2D), "hipMemcpy")
+        return 0
+
+    return provider
+
+
+def device_provider(force_buf, energy_buf=None):
+    """A provider that copies fixed device buffers (n x 3 float32 forces, nsys float64
+    energies) into the view on the stream: capturable, so it also runs under graphs."""
+
+    def provider(view, stream):
+        H = hip()
+        if force_buf.nbytes:
+            _hipcheck(H.hipMemcpyAsync(view.force, force_buf.ptr, force_buf.nbytes, D2D, stream), "hipMemcpyAsync")
+        if energy_buf is not None and energy_buf.nbytes:
+            _hipcheck(H.hipMemcpyAsync(view.energy, energy_buf.ptr, energy_buf.nbytes, D2D, stream), "hipMemcpyAsync")
+        return 0
+
+    return provider
 
 
 # ---------------------------------------------------------------------------------- C model

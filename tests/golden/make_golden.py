@@ -30,8 +30,20 @@ Outputs
         The ApoA1-layout unit-coefficient tables
         (Range_Limited_demo/SourceCode/CellMemoryModules/ApoA1_Mem_Ini_File/), which the
         library loads at run time.
+  tests/golden/cells_LJArgon_64.tar.gz
+        The 64-cell demo's cell-memory initialisation files (4 x 4 x 4 cells of 8.5 A):
+        CellMemoryModules/LJArgon_v_File_64_Cells/cell_ini_file_X_Y_Z.hex and
+        cell_ini_boundary_value_64_Cells.{mif,hex}.
+  tests/golden/cells_LJArgon_Mem_Ini.tar.gz
+        CellMemoryModules/LJArgon_Mem_Ini_File/cell_ini_file_X_Y_Z.{mif,hex},
+        cell_ini_boundary_value.{mif,hex}, and the .mif/.hex/.txt of one truncated debug
+        cell (LJArgon_debug_ini_File_64_Cells/cell_ini_file_debug_1_1_1.*).
+  tests/golden/cells_ApoA1.tar.gz
+        Two ApoA1 cells (CellMemoryModules/ApoA1_Mem_Ini_File/cell_ini_file_{2_2_2,3_3_3}.hex,
+        290-word memories) and cell_ini_boundary_value.hex.
   tests/golden/PROVENANCE.json
-        source path, SHA-256 and size of every file above.
+        source path, SHA-256 and size of every file above (tar members as
+        <archive>:<member>).
 """
 import gzip
 import hashlib
@@ -39,6 +51,7 @@ import json
 import os
 import shutil
 import sys
+import tarfile
 
 REF = "/root/reference"
 D = os.path.join(REF, "Range_Limited_demo", "SourceCode")
@@ -59,6 +72,19 @@ def sha256(path):
 def gz_copy(src, dst):
     with open(src, "rb") as fi, gzip.GzipFile(dst, "wb", mtime=0) as fo:
         shutil.copyfileobj(fi, fo)
+
+
+def tar_copy(srcs, dst):
+    """A reproducible .tar.gz of files (flat names, fixed owner and times)."""
+    with gzip.GzipFile(dst, "wb", mtime=0) as gz:
+        with tarfile.open(fileobj=gz, mode="w", format=tarfile.USTAR_FORMAT) as tar:
+            for src in sorted(srcs, key=os.path.basename):
+                info = tarfile.TarInfo(os.path.basename(src))
+                info.size = os.path.getsize(src)
+                info.mode = 0o644
+                info.mtime = 0
+                with open(src, "rb") as f:
+                    tar.addfile(info, f)
 
 
 def main():
@@ -116,6 +142,32 @@ def main():
             dst = os.path.join(TABLE_DIR, name)
             shutil.copyfile(src, dst)
             record(src, dst)
+
+    cm = os.path.join(D, "CellMemoryModules")
+
+    def cells(archive, srcs):
+        dst = os.path.join(HERE, archive)
+        tar_copy(srcs, dst)
+        for src in srcs:
+            prov["%s:%s" % (os.path.relpath(dst, REPO), os.path.basename(src))] = {
+                "source": os.path.relpath(src, REF), "sha256": sha256(src),
+                "bytes": os.path.getsize(src)}
+
+    v64 = os.path.join(cm, "LJArgon_v_File_64_Cells")
+    cells("cells_LJArgon_64.tar.gz",
+          [os.path.join(v64, "cell_ini_file_%d_%d_%d.hex" % (x, y, z))
+           for x in range(1, 5) for y in range(1, 5) for z in range(1, 5)] +
+          [os.path.join(v64, "cell_ini_boundary_value_64_Cells." + e) for e in ("mif", "hex")])
+    mem = os.path.join(cm, "LJArgon_Mem_Ini_File")
+    dbg = os.path.join(cm, "LJArgon_debug_ini_File_64_Cells")
+    cells("cells_LJArgon_Mem_Ini.tar.gz",
+          sorted(os.path.join(mem, f) for f in os.listdir(mem)
+                 if f.startswith("cell_ini_") and f.endswith((".mif", ".hex"))) +
+          [os.path.join(dbg, "cell_ini_file_debug_1_1_1." + e) for e in ("mif", "hex", "txt")])
+    apo = os.path.join(cm, "ApoA1_Mem_Ini_File")
+    cells("cells_ApoA1.tar.gz",
+          [os.path.join(apo, f) for f in ("cell_ini_file_2_2_2.hex", "cell_ini_file_3_3_3.hex",
+                                          "cell_ini_boundary_value.hex")])
 
     with open(os.path.join(HERE, "PROVENANCE.json"), "w") as f:
         json.dump(prov, f, indent=1, sort_keys=True)

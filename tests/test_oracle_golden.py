@@ -274,6 +274,21 @@ def test_fix_rounding_and_saturation():
     assert O.fix(-1e30) == -(2 ** 62)
 
 
+def test_lr_fix_rounding_and_saturation():
+    """Outside forces (kJ/mol/A) in the accumulator unit: 10 * 2^32 per kJ/mol/A."""
+    f = np.array([1.0, -0.1, 2.0 ** -34, 3 * 2.0 ** -34, -(2.0 ** -34), 5 * 2.0 ** -35,
+                  1e15, -1e15, np.nan, 0.0], np.float32)
+    got = O.lr_fix(f)
+    assert got[0] == 10 * 2 ** 32
+    assert got[1] == round(float(np.float32(-0.1)) * 10 * 2 ** 32)
+    assert got[2] == 2 and got[3] == 8 and got[4] == -2     # 2.5 -> 2, 7.5 -> 8, -2.5 -> -2
+    assert got[5] == round(5 * 10 / 8)                      # 6.25 -> 6
+    assert got[6] == 2 ** 62 and got[7] == -(2 ** 62) and got[8] == 0 and got[9] == 0
+    # where f * 10 is exact in FP32 it is the pair-force conversion of f * 10 (kJ/mol/nm)
+    for v in (0.75, -3.125, 1024.5, 2.0 ** -20):
+        assert O.lr_fix(np.float32(v)) == O.fix(np.float32(v * 10))
+
+
 def test_motion_update_closed_forms():
     """Free flight: x' = x + v dt.  Constant force: kick-drift and velocity Verlet both give
     v_n = v_0 + n a dt; VV positions follow x0 + v0 t + a t^2 / 2."""
