@@ -176,12 +176,13 @@ __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const
     const bool inw = wi < (unsigned)nwin;
     const int li = inw ? (int)wi : 0;
     if (STR == 4) {
-        const float4 v = *reinterpret_cast<const float4 *>(s_tab + li * 4);
+        const float4 v = *reinterpret_cast<const float4 *>(s_tab + (li << 2));
         c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
     } else {
-        const float2 u = *reinterpret_cast<const float2 *>(s_tab + li * 6);
-        const float2 v = *reinterpret_cast<const float2 *>(s_tab + li * 6 + 2);
-        const float2 w = *reinterpret_cast<const float2 *>(s_tab + li * 6 + 4);
+        const float *e = s_tab + __umul24((unsigned)li, 6u);   // 24-bit multiply: full rate
+        const float2 u = *reinterpret_cast<const float2 *>(e);
+        const float2 v = *reinterpret_cast<const float2 *>(e + 2);
+        const float2 w = *reinterpret_cast<const float2 *>(e + 4);
         c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y; c[4] = w.x; c[5] = w.y;
     }
     if (!inw) {

@@ -27,7 +27,7 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // neighbour (l mod APW) + (l / APW) + s*64/APW (mod m) against all its home atoms, so within
 // a step both indices of the queued pairs are distinct and one LDS read feeds every chunk;
 // steps are strided over the 12 waves.  The r2 <= rc2 test of D/Filter_Logic.v:234 is a
-// ballot; passing pairs are compacted into a per-wave LDS ring (the filter buffers and
+// ballot; passing pairs are compacted into a per-wave LDS queue (the filter buffers and
 // arbiter of D/Filter_Bank.v:548-606) and, whenever 64 are queued, all 64 lanes evaluate
 // them densely: table force from an LDS window, then the reference particle gets +F and the
 // neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed point
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float4 s_i[TI];
     __shared__ float4 s_j[TJ];
     __shared__ int s_it[TI];
-    __shared__ int s_jt[TJ];
+    __shared__ float s_jq[TJ];               // neighbour-tile charges (s_j[k].w holds k's queue bits)
     __shared__ unsigned long long s_iacc[3 * TI];
     __shared__ unsigned long long s_jacc[3 * TJ];
     __shared__ unsigned s_q[NWAVE][QCAP];
@@ -116,8 +116,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
     unsigned *q = s_q[wave];
     unsigned flag = 0;
-    unsigned long long npairs = 0, ncand = 0, nexcl = 0;
-    const unsigned rc2_bits = __float_as_uint(a.rc2);
+    unsigned long long npairs = 0, ncand = 0, ndrop = 0;
+    const float rc2 = a.rc2;
 
     for (int it = 0;; ++it) {
         int task;
@@ -211,8 +211,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const int sl = locate(cur, tid, sh);
                 float4 p = a.pos[sl];
                 if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
+                s_jq[tid] = p.w;
+                p.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)a.type[sl] << 16));
                 s_j[tid] = p;
-                s_jt[tid] = a.type[sl];
                 s_jslot[0][tid] = sl;
                 if (EXC) s_jo[tid] = a.orig[sl];
             }
@@ -236,23 +237,26 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const float4 csh = s_cshift[cur.o];   // the tile's shift (single-cell tiles)
                 const float sx = csh.x, sy = csh.y, sz = csh.z;
 
-                // queue entry: home index | neighbour index << 8 | type pair << 16
+                // queue entry: home index + neighbour index << 8 + type pair << 16.  The scan
+                // queues r2 <= rc2; a pair at r2 == 0 (coincident atoms, outside the
+                // reference's 0 < r2 filter) or an excluded pair is dropped here, uncounted.
                 auto eval = [&](unsigned e) {
                     const int ia = (int)(e & 0xffu), jb = (int)((e >> 8) & 0xffu);
                     const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
                     const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
                     const float2 ab = s_ab[e >> 16];
-                    const float qq = pa.w * pb.w;
+                    const float qq = pa.w * s_jq[jb];
                     float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
                                                                a.off, a.nseg, a.win_lo, a.nwin);
+                    bool drop = r2 == 0.0f;
                     if (EXC) {
                         const int2 om = s_iom[ia];
                         const int d = s_jo[jb] - om.x + 16;
-                        const bool ex = (unsigned)d < 32u && (((unsigned)om.y >> d) & 1u);
-                        nexcl += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(ex));
-                        F = ex ? 0.0f : F;
+                        drop = drop || ((unsigned)d < 32u && (((unsigned)om.y >> d) & 1u));
                     }
+                    ndrop += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(drop));
+                    F = drop ? 0.0f : F;
                     const float fx = F * dx, fy = F * dy, fz = F * dz;
                     if (F32) {
                         atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
@@ -273,7 +277,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     }
                 };
 
-                unsigned head = 0, tail = 0;
+                // the wave's queue holds entries q[0, tail), tail < 64 between tests
+                unsigned tail = 0;
                 const int bj = il % mj;
                 const int nsteps = (mj + P - 1) / P;
                 for (int cg0 = 0; cg0 < mi; cg0 += NCHUNK * APW) {
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     for (int ch = 0; ch < NCHUNK; ++ch) {
                         const int ia = cg0 + ch * APW + il;
                         pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
-                        eb[ch] = (unsigned)ia | ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << 16);
+                        eb[ch] = (unsigned)ia + ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << 16);
                     }
                     // SELF: the home cell, where a pair needs jb > itile + ia - jtile
                     // (compiled twice so the other 13 offsets skip that test)
@@ -302,23 +307,30 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                             jb = (jb >= mj) ? jb - mj : jb;
                             jb = kv ? jb : 0;
                             const float4 pj = s_j[jb];
-                            const unsigned jsh = ((unsigned)jb << 8) | ((unsigned)s_jt[jb] << 16);
+                            const unsigned jsh = __float_as_uint(pj.w);
 #pragma unroll
                             for (int ch = 0; ch < NCHUNK; ++ch) {
                                 if (ch < nch) {
                                     float dx, dy, dz;
                                     const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
-                                    // 0 < r2 <= rc2 as one unsigned compare of the bit patterns (r2 >= +0)
-                                    const bool pass = kv && (__float_as_uint(r2) - 1u < rc2_bits) &&
-                                                      (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
+                                    const bool pass = kv && (r2 <= rc2) && (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
                                     const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
-                                    if (pass) q[(tail + lane_rank(m)) & (QCAP - 1)] = eb[ch] | jsh;
+                                    // slot = tail + the lane's rank among the passing lanes
+                                    if (pass)
+                                        q[__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, tail))] = eb[ch] + jsh;
                                     tail += (unsigned)__popcll(m);
-                                    if (tail - head >= 64u) {
+                                    if (tail >= 64u) {
+                                        // evaluate the first 64, move the rest (< 64) to the front
                                         __builtin_amdgcn_wave_barrier();
-                                        eval(q[(head + lane) & (QCAP - 1)]);
+                                        eval(q[lane]);
+                                        const unsigned rest = tail - 64u;
+                                        const unsigned mv = q[64 + lane];
                                         __builtin_amdgcn_wave_barrier();
-                                        head += 64u;
+                                        if ((unsigned)lane < rest) q[lane] = mv;
+                                        __builtin_amdgcn_wave_barrier();
+                                        npairs += 64u;
+                                        tail = rest;
                                     }
                                 }
                             }
@@ -329,9 +341,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
                 }
                 npairs += tail;
-                if (tail != head) {
+                if (tail) {
                     __builtin_amdgcn_wave_barrier();
-                    if ((unsigned)lane < tail - head) eval(q[(head + lane) & (QCAP - 1)]);
+                    if ((unsigned)lane < tail) eval(q[lane]);
                     __builtin_amdgcn_wave_barrier();
                 }
                 lds_barrier();
@@ -360,8 +372,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     float4 sh;   // slot and shift again from LDS rather than held in registers
                     const int sl = locate(nxt, tid, sh);
                     if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
+                    s_jq[tid] = pf.w;
+                    pf.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)tf << 16));
                     s_j[tid] = pf;
-                    s_jt[tid] = tf;
                     s_jslot[jpar ^ 1][tid] = sl;
                     if (EXC) s_jo[tid] = of;
                 }
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     }
     // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
-        s_stat[wave][0] = npairs - nexcl;
+        s_stat[wave][0] = npairs - ndrop;
         s_stat[wave][1] = ncand * 64ull;
     }
     lds_barrier();

@@ -135,6 +135,35 @@ def test_small_boxes_all_images(apoa1):
         assert st.pair_evals == nb, "replica %d (L=%.2f)" % (index, s.box[0])
 
 
+@pytest.mark.parametrize("nt", [3, 5, 7])
+@pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])
+def test_species_counts_not_powers_of_two(apoa1, nt, mode):
+    """Type-pair multipliers for 3, 5 and 7 species (the queue entry adds type_i * ntypes
+    and type_j, which only an OR would confuse for non-powers of two)."""
+    s = mdrl.SyntheticSystem(4, 3)
+    rng = np.random.default_rng(nt)
+    types = rng.integers(0, nt, size=s.n).astype(np.int32)
+    eps = np.linspace(0.2, 0.7, nt).astype(np.float32)
+    sig = np.linspace(1.0, 3.5, nt).astype(np.float32)
+    mass = np.full(nt, 12.011, np.float32)
+    say("species: %d types, n=%d, mode %s" % (nt, s.n, mode))
+    flags = {"fast": 0, "parity": mdrl.EVAL_PARITY, "analytic": mdrl.EVAL_ANALYTIC}[mode]
+    omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
+    g = mdrl.System(s.n, s.box, s.rc, eps, sig, mass, dt=s.dt, coulomb_k=s.coulomb_k, flags=flags)
+    g.set_types(types, s.charge)
+    g.set_particles(s.particle)
+    got = g.forces_fixed()
+    npairs = g.stats().pair_evals
+    g.close()
+    ncell = [int(math.floor(float(s.box[d]) / s.rc)) for d in range(3)]
+    rl = O.RL(s.particle[:, :3], s.box, ncell, s.rc, types, eps, sig, apoa1_tab6(apoa1),
+              charge=s.charge, coulomb_k=s.coulomb_k, mode=omode)
+    want, wpairs = rl.forces_fix()
+    assert npairs == wpairs
+    bad = np.nonzero(np.any(got != want, axis=1))[0]
+    assert bad.size == 0, "atoms differ: %s" % bad[:10]
+
+
 # ------------------------------------------------------------------------ steps
 @pytest.mark.parametrize("which,index,nsteps", [(1, 0, 20), (4, 5, 10), (2, 0, 3)])
 def test_steps_bit_exact(apoa1, which, index, nsteps):
