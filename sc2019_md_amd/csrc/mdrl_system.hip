@@ -856,7 +856,7 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
 }
 
 // Each slot takes the next place of its cell's share for the copy its block counted into
-// (the order inside a cell is fixed afterwards by k_cell_sort).
+// (the order inside a cell is fixed afterwards by k_cell_gather).
 __global__ void k_scatter(int n, const int *cell, int *fill, int *perm, int ncells)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -865,50 +865,51 @@ __global__ void k_scatter(int n, const int *cell, int *fill, int *perm, int ncel
     if (s < n) perm[k] = s;
 }
 
-// Orders each cell's slots by input index.  Cells larger than the LDS capacity keep
-// the scatter order and raise MDRL_F_CELL_OVERFLOW (forces stay correct in int64 mode).
+// One workgroup per cell: orders the cell's slots by input index and moves their particle
+// data into the new slots, clearing the forces and recording each input index's slot.
+// A cell larger than the LDS capacity keeps the scatter order and raises
+// MDRL_F_CELL_OVERFLOW (forces stay correct in int64 mode).
 constexpr int SORT_CAP = 2048;
-__global__ __launch_bounds__(256) void k_cell_sort(const int *start, int *perm, const int *orig,
-                                                   unsigned long long *counters)
+__global__ __launch_bounds__(256) void k_cell_gather(const int *start, const int *perm, const float4 *pos,
+                                                     const float4 *vel, const int *type, const int *orig,
+                                                     float4 *pos2, float4 *vel2, int *type2, int *orig2,
+                                                     long long *fix, float *f32, int *slot_of,
+                                                     unsigned long long *counters)
 {
     __shared__ int s_v[SORT_CAP];
     __shared__ int s_k[SORT_CAP];
     const int c = blockIdx.x;
     const int b = start[c], m = start[c + 1] - b;
-    if (m <= 1) return;
+    auto move = [&](int s, int o, int io) {
+        pos2[s] = pos[o];
+        vel2[s] = vel[o];
+        type2[s] = type[o];
+        orig2[s] = io;
+        if (slot_of) slot_of[io] = s;
+        if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
+        if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    };
     if (m > SORT_CAP) {
         if (threadIdx.x == 0)
             __hip_atomic_fetch_or(&counters[2], 4ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const int o = perm[b + k];
+            move(b + k, o, orig[o]);
+        }
         return;
     }
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        int v = perm[b + k];
+        const int v = perm[b + k];
         s_v[k] = v;
         s_k[k] = orig[v];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        int key = s_k[k], r = 0;
+        const int key = s_k[k];
+        int r = 0;
         for (int u = 0; u < m; ++u) r += (s_k[u] < key);
-        perm[b + r] = s_v[k];
+        move(b + r, s_v[k], key);
     }
-}
-
-__global__ void k_gather(int n, const int *perm, const float4 *pos, const float4 *vel,
-                         const int *type, const int *orig, float4 *pos2, float4 *vel2,
-                         int *type2, int *orig2, long long *fix, float *f32, int *slot_of)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int o = perm[s];
-    pos2[s] = pos[o];
-    vel2[s] = vel[o];
-    type2[s] = type[o];
-    const int io = orig[o];
-    orig2[s] = io;
-    if (slot_of) slot_of[io] = s;
-    if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
-    if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 int launch_bin_initial(DeviceSystem &s, hipStream_t st)
@@ -927,12 +928,10 @@ int launch_rebin(DeviceSystem &s, hipStream_t st)
     if (s.n > 0) {
         hipLaunchKernelGGL(k_scatter, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.cell,
                            s.cell_fill, s.perm, s.ncells);
-        hipLaunchKernelGGL(k_cell_sort, dim3(s.ncells), dim3(256), 0, st, s.cell_start, s.perm,
-                           s.orig, s.counters);
-        hipLaunchKernelGGL(k_gather, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.perm, s.pos,
-                           s.vel, s.type, s.orig, s.pos2, s.vel2, s.type2, s.orig2,
+        hipLaunchKernelGGL(k_cell_gather, dim3(s.ncells), dim3(256), 0, st, s.cell_start, s.perm,
+                           s.pos, s.vel, s.type, s.orig, s.pos2, s.vel2, s.type2, s.orig2,
                            (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix,
-                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr, s.slot_of);
+                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr, s.slot_of, s.counters);
     }
     MDRL_HIP(hipGetLastError());
     std::swap(s.pos, s.pos2);
