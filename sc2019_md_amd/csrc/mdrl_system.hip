@@ -811,46 +811,67 @@ __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const in
 // Exclusive scan of the next-step counts (kCountCopies copies per cell, summed) into
 // cell_start, and the start of each copy's share of each cell into the fill pointers;
 // clears the counts; records the largest cell; clears the pair counters and the work
-// queues of the force evaluation that follows.  One workgroup of 1024 threads.
+// queues of the force evaluation that follows.  One workgroup of 1024 threads walks the
+// cells in tiles of 1024, one cell per thread, so every load and store is coalesced and
+// each thread's kCountCopies loads are independent.
 __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *start, int *fill,
                                                unsigned long long *counters)
 {
     __shared__ int s_sum[1024];
-    __shared__ int s_max[1024];
+    __shared__ int s_wsum[16];
+    __shared__ int s_run;
     const int t = threadIdx.x;
-    const int per = (ncells + 1023) / 1024;
-    const int b = t * per, e = min(ncells, b + per);
-    int sum = 0, mx = 0;
-    for (int c = b; c < e; ++c) {
-        int v = 0;
-        for (int k = 0; k < kCountCopies; ++k) v += count[(size_t)k * ncells + c];
-        sum += v;
-        mx = max(mx, v);
-    }
-    s_sum[t] = sum;
-    s_max[t] = mx;
+    if (t == 0) s_run = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? s_sum[t - off] : 0;
-        int m = (t >= off) ? s_max[t - off] : 0;
-        __syncthreads();
-        s_sum[t] += v;
-        s_max[t] = max(s_max[t], m);
-        __syncthreads();
-    }
-    int run = s_sum[t] - sum;
-    for (int c = b; c < e; ++c) {
-        start[c] = run;
+    int mx = 0;
+    for (int base = 0; base < ncells; base += 1024) {
+        const int c = base + t;
+        int v[kCountCopies];
+        int tot = 0;
+#pragma unroll
         for (int k = 0; k < kCountCopies; ++k) {
-            const size_t q = (size_t)k * ncells + c;
-            fill[q] = run;
-            run += count[q];
-            count[q] = 0;
+            v[k] = (c < ncells) ? count[(size_t)k * ncells + c] : 0;
+            tot += v[k];
         }
+        mx = max(mx, tot);
+        // inclusive scan of the tile: within each wave by shuffles, then across the 16
+        // wave totals
+        const int lane = t & 63, w = t >> 6;
+        int inc = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(inc, off);
+            if (lane >= off) inc += u;
+        }
+        if (lane == 63) s_wsum[w] = inc;
+        __syncthreads();
+        int before = s_run;
+        for (int k = 0; k < w; ++k) before += s_wsum[k];
+        if (c < ncells) {
+            int r = before + inc - tot;
+            start[c] = r;
+#pragma unroll
+            for (int k = 0; k < kCountCopies; ++k) {
+                const size_t q = (size_t)k * ncells + c;
+                fill[q] = r;
+                r += v[k];
+                count[q] = 0;
+            }
+        }
+        __syncthreads();
+        if (t == 1023) s_run = before + inc;
+        __syncthreads();
     }
-    if (t == 1023) {
-        start[ncells] = s_sum[1023];
-        counters[3] = (unsigned long long)s_max[1023];
+    // largest cell
+    s_sum[t] = mx;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off) s_sum[t] = max(s_sum[t], s_sum[t + off]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        start[ncells] = s_run;
+        counters[3] = (unsigned long long)s_sum[0];
     }
     if (t < 2 * kStatSlots + kQueues) counters[8 + t] = 0ull;
 }
