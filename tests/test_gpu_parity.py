@@ -49,6 +49,39 @@ def test_eval_pairs_golden_bit_exact(table):
     assert np.array_equal(fs.view(np.uint32), hex2f(recs[:, 6]).view(np.uint32))
 
 
+def test_eval_pairs_testbench_vectors(table, apoa1):
+    """The known answers of the reference's r2 and pair-evaluation testbenches through the
+    pair API, against the oracle (whose r2 and d reproduce the testbench comments)."""
+    from tests.test_oracle_golden import TB_NBR, TB_REF
+    want = O.pair_golden(*apoa1[mdrl.TERM_14], *apoa1[mdrl.TERM_8], TB_REF, TB_NBR, 196.0)
+    fs, fx = mdrl.eval_pairs_ex(table, TB_REF, TB_NBR, 196.0, mdrl.PAIR_PARITY | mdrl.PAIR_STRICT)
+    assert np.array_equal(fs.view(np.uint32), want[:, 0].view(np.uint32))
+    assert np.array_equal(fx.view(np.uint32), want[:, 1:4].view(np.uint32))
+
+
+def test_testbench_periodic_vector(apoa1):
+    """r2_compute_with_pbc_tb.v:129-140: ref (59, 50.5, 50.5) and nbr (0, 0, 0) in the
+    59.5 x 51 x 51 A box of the LJArgon-20k data give d = (-0.5, -0.5, -0.5), r2 = 0.75:
+    as a two-atom system (rc 8.5 A, 7 x 6 x 6 cells) the pair is found through the periodic
+    image, once, with forces equal to the oracle's and opposite on the two atoms."""
+    box = np.array([59.5, 51.0, 51.0], np.float32)
+    p = np.zeros((2, 11), np.float32)
+    p[0, :3] = [59.0, 50.5, 50.5]
+    eps, sig, mass = [0.996], [1.0], [39.948]     # r = 0.87 A < 2^(1/6) sigma: repulsive
+    g = mdrl.System(2, box, 8.5, eps, sig, mass, dt=2.0)
+    g.set_types(np.zeros(2, np.int32))
+    g.set_particles(p)
+    got = g.forces_fixed()
+    npairs = g.stats().pair_evals
+    g.close()
+    rl = O.RL(p[:, :3], box, [7, 6, 6], 8.5, np.zeros(2, np.int32), eps, sig, apoa1_tab6(apoa1))
+    want, wpairs = rl.forces_fix()
+    assert npairs == wpairs == 1
+    assert np.array_equal(got, want)
+    # repulsion pushes the atom at the upper faces away from the image across them
+    assert np.all(got[0] == -got[1]) and got[0, 0] == got[0, 1] == got[0, 2] and got[0, 0] < 0
+
+
 def test_eval_pairs_fast_mode_close(table):
     """Fused arithmetic differs from the golden records by rounding only."""
     recs = parse_reference_output()

@@ -265,6 +265,21 @@ def test_two_atom_table_force(apoa1):
     assert f[1, 0] == -f[0, 0]
 
 
+# Known answers of the reference's testbenches: r2_compute_with_pbc_tb.v:70-129 and
+# RL_LJ_Evaluate_Pairs_1st_Order_tb.v:87-104 (d = ref - nbr and r2 in the comments).
+TB_REF = np.array([[2, 4, 8], [2, 2, 2], [1, 1, 1], [1, 2, 4], [2, 2, 2]], np.float32)
+TB_NBR = np.array([[1, 1, 1], [1, 1, 1], [2, 4, 8], [0, 0, 0], [0, 0, 0]], np.float32)
+TB_R2 = ["426C0000", "40400000", "426C0000", "41A80000", "41400000"]     # 59, 3, 59, 21, 12
+TB_D = [["3F800000", "40400000", "40E00000"], ["3F800000"] * 3, ["BF800000", "C0400000", "C0E00000"],
+        ["3F800000", "40000000", "40800000"], ["40000000"] * 3]
+
+
+def test_testbench_r2_vectors(apoa1):
+    out = O.pair_golden(*apoa1[mdrl.TERM_14], *apoa1[mdrl.TERM_8], TB_REF, TB_NBR, 196.0)
+    assert np.array_equal(out[:, 4].view(np.uint32), hex2f(TB_R2).view(np.uint32))
+    assert np.array_equal(out[:, 5:8].view(np.uint32), hex2f(sum(TB_D, [])).reshape(5, 3).view(np.uint32))
+
+
 def test_fix_rounding_and_saturation():
     assert O.fix(1.0) == 2 ** 32
     assert O.fix(-0.25) == -(2 ** 30)
