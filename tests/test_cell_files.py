@@ -148,6 +148,19 @@ def test_debug_cell_formats_agree(meminit, ljargon_lists):
     assert np.array_equal(a.view(np.uint32), lists[(1, 1, 1)][:13].view(np.uint32))
 
 
+def test_large_cell_mif_round_trip(tmp_path):
+    """Counts of 256 and more: the generator's %X word 0 grows to 25 hex digits (a value
+    still equal to the count); the reader takes it right-aligned."""
+    rng = np.random.default_rng(4)
+    xyz = rng.uniform(0, 12, (300, 3)).astype(np.float32)
+    for ext in (".mif", ".hex"):
+        path = str(tmp_path / ("cell_ini_file_1_1_1" + ext))
+        mdrl.write_cell_file(path, xyz, depth=400)
+        assert np.array_equal(mdrl.read_cell_file(path).view(np.uint32), xyz.view(np.uint32))
+    word0 = open(str(tmp_path / "cell_ini_file_1_1_1.mif")).read().splitlines()[6]
+    assert word0 == "0 : 000000000000000000000012C;"
+
+
 def test_malformed_files_rejected(demo64, tmp_path):
     src = open(os.path.join(demo64, "cell_ini_file_1_1_1.hex")).read()
     bad = tmp_path / "cell_ini_file_9_9_9.hex"

@@ -107,6 +107,25 @@ def test_conversion_ties_and_saturation(apoa1):
     assert np.array_equal(got, fix + O.lr_fix(f))
 
 
+def test_fp32_accumulation_mode():
+    """With FP32 atomics the provider's forces are added as f * 10 in FP32: the forces
+    stay within 1e-5 (relative L2) of the int64 run with the same provider."""
+    s = mdrl.SyntheticSystem(4, 3)
+    f = (s.charge.astype(np.float32)[:, None] * FIELD).astype(np.float32)
+    buf = mdrl.DeviceBuffer(f)
+    out = []
+    for flags, hook in ((0, True), (mdrl.ACC_FP32, True), (mdrl.ACC_FP32, False)):
+        g = mdrl.System.from_synthetic(s, flags=flags)
+        if hook:
+            g.set_long_range(mdrl.device_provider(buf))
+        out.append(g.forces_fixed().astype(np.float64))
+        g.close()
+    assert np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0]) < 1e-5
+    # the provider's share in FP32 mode: the field forces, up to the FP32 atomics' order
+    lr = O.lr_fix(f).astype(np.float64)
+    assert np.linalg.norm((out[1] - out[2]) - lr) / np.linalg.norm(lr) < 1e-2
+
+
 def test_steps_bit_exact(apoa1):
     s = mdrl.SyntheticSystem(4, 11)
     nsteps = 6
