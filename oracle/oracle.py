@@ -292,7 +292,10 @@ def lr_fix(f):
     accumulators: rint(f * 10 * 2^32), ties to even, saturated at +-2^62, NaN to 0.  The
     reference adds its short-range, long-range and bonded forces in FP32 per particle
     (Summation_Logic/Force_Summation_Unit.v:62-126, long-range + short-range first, then
-    bonded); here the three meet in int64, where the sum does not depend on the order."""
+    bonded); here the three meet in int64, where the sum does not depend on the order.
+    Pinned by exact values (tests/test_oracle_golden.py::test_lr_fix_rounding_and_saturation):
+    ties to even, saturation, NaN, and equality with the pair-force conversion of f * 10
+    wherever that product is exact in FP32."""
     x = np.asarray(f, np.float32).astype(np.float64) * 42949672960.0
     x = np.where(np.isnan(x), 0.0, np.clip(x, -2.0 ** 62, 2.0 ** 62))
     return np.rint(x).astype(np.int64)
@@ -312,7 +315,8 @@ def generator_cell_lists(pos, edge, ncell, rule="floor"):
     cell = floor(x / edge) + 1) and M/GenInputCellList_ApoA1.m:103-133 (rule "ceil":
     ceil(x / edge)), per coordinate, FP32 division, a coordinate of exactly 0 in cell 1;
     particles outside 1..ncell are dropped.  Returns ({(X, Y, Z): k x 3 FP32 positions in
-    input order}, number dropped)."""
+    input order}, number dropped).  Pinned by the reference's own output of this script: all
+    92 LJArgon cell files shipped in D/CellMemoryModules (tests/test_cell_files.py)."""
     pos = np.asarray(pos, np.float32).reshape(-1, 3)
     e = np.float32(edge)
     lists, dropped = {}, 0
