@@ -132,3 +132,38 @@ def test_batch_energies_deterministic():
                                      charge=s.charge, ke=s.coulomb_k)
         assert e1[k][0] == pytest.approx(e64, rel=2e-4, abs=1e-3 * abs(e64) + 1.0)
     b.close()
+
+
+def test_batch_with_empty_and_tiny_systems(apoa1):
+    """A batch may hold systems with no atom, one atom or two atoms beside ordinary ones:
+    the ordinary systems keep the bits of their single-system runs, the lone atom feels no
+    force from its own images (box > rc), the two atoms feel equal and opposite forces, and
+    the empty system reports no atoms and zero energy."""
+    big = mdrl.SyntheticSystem(4, 3)
+    s0 = big
+    eps, sig, mass = s0.eps, s0.sigma, s0.mass
+    sizes = [s0.n, 0, 1, 2]
+    boxes = [s0.box, [22.0, 22.0, 22.0], [25.0, 25.0, 25.0], [21.0, 21.0, 21.0]]
+    b = mdrl.Batch(sizes, boxes, s0.rc, eps, sig, mass, dt=s0.dt, coulomb_k=s0.coulomb_k)
+    b.set_system(0, s0.particle, s0.type, s0.charge)
+    one = np.zeros((1, 11), np.float32)
+    one[0, :3] = [3.0, 4.0, 5.0]
+    b.set_system(2, one, np.zeros(1, np.int32), np.zeros(1, np.float32))
+    two = np.zeros((2, 11), np.float32)
+    two[0, :3] = [20.5, 10.0, 10.0]
+    two[1, :3] = [2.0, 10.5, 10.0]                   # 2.5 A away through the x faces
+    b.set_system(3, two, np.array([1, 2], np.int32), np.array([0.3, -0.3], np.float32))
+    b.commit()
+    b.step(4)
+    g = mdrl.System.from_synthetic(s0)
+    g.step(4)
+    assert np.array_equal(b.system_particles(0).view(np.uint32), g.particles().view(np.uint32))
+    assert np.array_equal(b.system_forces_fixed(0), g.forces_fixed())
+    g.close()
+    assert b.system_particles(1).shape == (0, 11)
+    pe, ke, _ = b.system_energy(1)
+    assert pe == 0.0 and ke == 0.0
+    assert np.all(b.system_forces_fixed(2) == 0)
+    f2 = b.system_forces_fixed(3)
+    assert np.all(f2[0] == -f2[1]) and np.any(f2 != 0)
+    b.close()
