@@ -61,7 +61,26 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libmdrl.so failed")
     os.replace(tmp, OUT)
+    build_host_example()
     return OUT
+
+
+HOST_SRC = os.path.join(HERE, "examples", "host_c.c")
+HOST_OUT = os.path.join(HERE, "examples", "host_c")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_host_example(out=HOST_OUT, cc=None):
+    """The plain C host of examples/host_c.c, linked against libmdrl.so (found next to the
+    package through an $ORIGIN-relative rpath, so the pair can move together)."""
+    cc = cc or os.environ.get("CC", "gcc")
+    cmd = [cc, "-std=c99", "-O2", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", INCLUDE, HOST_SRC,
+           "-o", out, "-L", HERE, "-lmdrl", "-Wl,-rpath,$ORIGIN/.." if out == HOST_OUT else "-Wl,-rpath," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of %s failed" % os.path.basename(out))
+    return out
 
 
 if __name__ == "__main__":

@@ -75,3 +75,11 @@ def test_create_rejects_bad_parameters():
     with pytest.raises(mdrl.MdrlError) as e:
         mdrl.System(10, (10, 10, 10), 3.0, [1.0] * 9, [1.0] * 9, [1.0] * 9)
     assert e.value.code == -1
+
+
+def test_c_host_builds_against_the_header(tmp_path):
+    """The plain C host (sc2019_md_amd/examples/host_c.c) compiles as strict C99 with
+    -Wall -Wextra -pedantic -Werror against include/mdrl.h and links against libmdrl.so."""
+    import sc2019_md_amd.build as B
+    out = B.build_host_example(out=str(tmp_path / "host_c"))
+    assert os.path.exists(out) and os.access(out, os.X_OK)
