@@ -387,6 +387,9 @@ def main():
                     help="NVE steps of the precision leg (workload 3)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one device copy of this many bytes (traffic counter calibration)")
+    ap.add_argument("--long-range-field", action="store_true",
+                    help="attach a summation-hook provider (a uniform field on the charges, copied "
+                         "from a device buffer each evaluation) for the steps, to time the hook")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -413,6 +416,13 @@ def main():
     relaxed = g.particles()
 
     sched, tune_ms = g.autotune(reps=3)
+    field_buf = None
+    if args.long_range_field:
+        # F_i = q_i E with E = (0.8, -0.3, 0.45) kJ/mol/A/e, input order
+        field = (s.charge.astype(np.float32)[:, None] *
+                 np.array([0.8, -0.3, 0.45], np.float32)).astype(np.float32)
+        field_buf = mdrl.DeviceBuffer(field)
+        g.set_long_range(mdrl.device_provider(field_buf))
     g.step(args.warmup)
     # timed region: the steps as a user runs them (graph-captured unless --graph 0)
     g.set_graph(bool(args.graph))
@@ -480,6 +490,8 @@ def main():
         "dt_fs": s.dt,
         "step_device_ms_eager": st.step_ms,
         "graph_captured_steps": bool(args.graph),
+        "long_range_hook": ("uniform field on the charges, device copy per evaluation"
+                            if args.long_range_field else None),
         "energy_kj_mol": {"potential_initial": pe_start, "potential_relaxed": pe_relaxed,
                           "kinetic_relaxed": ke0, "potential_end": pe, "kinetic_end": ke},
         "pre_relaxation": {"iterations": relax, "max_disp_A": args.relax_step,
