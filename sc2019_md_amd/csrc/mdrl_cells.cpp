@@ -13,6 +13,7 @@
 // Boundary file (M/GenCellBoundary_LJArgon.m:48-60): 32-bit words, word k = k * edge.
 // .hex: one data record per word (":0C" or ":04", 16-bit word address, type 00,
 // data, two's-complement checksum), ":00000001FF" at the end.
+#include <ctype.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,21 +24,38 @@
 #include "mdrl_internal.h"
 
 using mdrl::set_error;
+using mdrl::WordFile;
 
 namespace {
-
-// Word file contents: nwords entries of `width` bits (32 or 96), each as 1 or 3 32-bit
-// parts, most significant part first.
-struct WordFile {
-    int width = 0;
-    int depth = 0;
-    std::vector<uint32_t> parts;   // depth x (width / 32)
-};
 
 bool ends_with(const char *s, const char *suffix)
 {
     const size_t a = strlen(s), b = strlen(suffix);
     return a >= b && strcmp(s + a - b, suffix) == 0;
+}
+
+int hex_digit(char c)
+{
+    if (c >= '0' && c <= '9') return c - '0';
+    if (c >= 'A' && c <= 'F') return c - 'A' + 10;
+    if (c >= 'a' && c <= 'f') return c - 'a' + 10;
+    return -1;
+}
+
+// Two hex digits at s as a byte, or -1 (no sign, space or "0x" accepted).
+int hex_byte(const char *s)
+{
+    const int hi = hex_digit(s[0]);
+    if (hi < 0) return -1;
+    const int lo = hex_digit(s[1]);
+    return lo < 0 ? -1 : (hi << 4) | lo;
+}
+
+bool blank(const char *s)
+{
+    for (; *s; ++s)
+        if (*s != ' ' && *s != '\t' && *s != '\r' && *s != '\n') return false;
+    return true;
 }
 
 // A hex string of any length, right-aligned into nparts 32-bit parts (most significant
@@ -64,96 +82,206 @@ bool parse_hex_value(const char *h, size_t len, int nparts, uint32_t *out)
     return true;
 }
 
+// Only white space up to the end of the line, or a "--" comment after it.
+bool is_blank_or_comment(const char *s)
+{
+    while (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\n') ++s;
+    return *s == 0 || (s[0] == '-' && s[1] == '-');
+}
+
+const char *skip_ws(const char *s)
+{
+    while (*s == ' ' || *s == '\t') ++s;
+    return s;
+}
+
+// Decimal digits at *s into v (no sign; at most 9 digits), advancing *s.
+bool read_dec(const char **s, long *v)
+{
+    const char *p = *s;
+    long x = 0;
+    int nd = 0;
+    for (; *p >= '0' && *p <= '9'; ++p, ++nd) {
+        if (nd == 9) return false;
+        x = 10 * x + (*p - '0');
+    }
+    if (nd == 0) return false;
+    *s = p;
+    *v = x;
+    return true;
+}
+
+// The identifier at *s (letters and '_'), advancing *s.
+std::string read_word(const char **s)
+{
+    const char *p = *s;
+    while ((*p >= 'A' && *p <= 'Z') || (*p >= 'a' && *p <= 'z') || *p == '_') ++p;
+    std::string w(*s, (size_t)(p - *s));
+    for (char &c : w) c = (char)toupper((unsigned char)c);
+    *s = p;
+    return w;
+}
+
+// Quartus memory initialisation file, the form the generators write: header lines
+// "KEY = value;" (DEPTH, WIDTH, ADDRESS_RADIX = DEC, DATA_RADIX = HEX), "CONTENT", "BEGIN",
+// then "addr : value;" or "[lo..hi] : value;" lines and "END;".  "--" starts a comment.
+// Anything else is an error, so a damaged file is not read as a different memory.
 int read_mif_words(FILE *f, const char *path, WordFile &w)
 {
     char line[1024];
-    bool content = false;
+    bool content = false, begun = false, addr_dec = false, data_hex = false, ended = false;
     while (fgets(line, sizeof line, f)) {
-        if (!content) {
-            int v;
-            if (sscanf(line, " DEPTH = %d", &v) == 1) w.depth = v;
-            else if (sscanf(line, " WIDTH = %d", &v) == 1) w.width = v;
-            else if (strncmp(line, "BEGIN", 5) == 0) {
-                if ((w.width != 32 && w.width != 96) || w.depth <= 0 || w.depth > (1 << 20)) {
-                    set_error("%s: unsupported WIDTH %d / DEPTH %d", path, w.width, w.depth);
-                    return MDRL_EIO;
+        if (is_blank_or_comment(line)) continue;
+        if (ended) { set_error("%s: text after END", path); return MDRL_EIO; }
+        const char *p = skip_ws(line);
+        if (!begun) {
+            const std::string key = read_word(&p);
+            if (key == "CONTENT" || key == "BEGIN") {
+                if (key == "CONTENT") {
+                    content = true;
+                    p = skip_ws(p);
+                    const char *q = p;
+                    if (read_word(&q) == "BEGIN") { begun = true; p = q; }
+                } else {
+                    begun = content;
                 }
-                w.parts.assign((size_t)w.depth * (w.width / 32), 0u);
-                content = true;
+                if (!begun && key == "BEGIN") { set_error("%s: BEGIN without CONTENT", path); return MDRL_EIO; }
+                if (!is_blank_or_comment(p)) { set_error("%s: malformed line '%.40s'", path, line); return MDRL_EIO; }
+                if (begun) {
+                    if (w.width != 32 && w.width != 96) { set_error("%s: unsupported WIDTH %d", path, w.width); return MDRL_EIO; }
+                    if (w.depth <= 0 || w.depth > (1 << 20)) { set_error("%s: unsupported DEPTH %d", path, w.depth); return MDRL_EIO; }
+                    if (!addr_dec || !data_hex) {
+                        set_error("%s: ADDRESS_RADIX = DEC and DATA_RADIX = HEX expected", path);
+                        return MDRL_EIO;
+                    }
+                    w.parts.assign((size_t)w.depth * (w.width / 32), 0u);
+                    w.given.assign((size_t)w.depth, 0);
+                }
+                continue;
             }
+            p = skip_ws(p);
+            if (content || *p != '=') { set_error("%s: malformed header line '%.40s'", path, line); return MDRL_EIO; }
+            p = skip_ws(p + 1);
+            long v = 0;
+            std::string val;
+            const bool numeric = key == "DEPTH" || key == "WIDTH";
+            if (numeric ? !read_dec(&p, &v) : (val = read_word(&p)).empty()) {
+                set_error("%s: malformed header line '%.40s'", path, line);
+                return MDRL_EIO;
+            }
+            p = skip_ws(p);
+            if (*p != ';' || !is_blank_or_comment(p + 1)) { set_error("%s: malformed header line '%.40s'", path, line); return MDRL_EIO; }
+            if (key == "DEPTH") w.depth = (int)v;
+            else if (key == "WIDTH") w.width = (int)v;
+            else if (key == "ADDRESS_RADIX" && val == "DEC") addr_dec = true;
+            else if (key == "DATA_RADIX" && val == "HEX") data_hex = true;
+            else { set_error("%s: unsupported header line '%.40s'", path, line); return MDRL_EIO; }
             continue;
         }
-        if (strncmp(line, "END", 3) == 0) return MDRL_OK;
-        char *p = line;
-        while (*p == ' ' || *p == '\t') ++p;
-        if (*p == '\r' || *p == '\n' || *p == 0 || *p == '-') continue;   // blank or comment
-        char *colon = strchr(p, ':');
-        char *semi = colon ? strchr(colon, ';') : nullptr;
-        if (!colon || !semi) { set_error("%s: malformed line '%.40s'", path, line); return MDRL_EIO; }
-        long lo, hi;
-        if (p[0] == '[') {                                  // [lo..hi] : value;
-            if (sscanf(p, "[%ld..%ld]", &lo, &hi) != 2) { set_error("%s: malformed range", path); return MDRL_EIO; }
-        } else {
-            char *end;
-            lo = hi = strtol(p, &end, 10);
-            if (end == p) { set_error("%s: malformed address '%.40s'", path, line); return MDRL_EIO; }
+        {
+            const char *q = p;
+            if (read_word(&q) == "END") {
+                q = skip_ws(q);
+                if (*q != ';' || !is_blank_or_comment(q + 1)) { set_error("%s: malformed END line", path); return MDRL_EIO; }
+                ended = true;
+                continue;
+            }
         }
-        const char *h = colon + 1;
-        while (*h == ' ' || *h == '\t') ++h;
-        size_t len = (size_t)(semi - h);
-        while (len > 0 && (h[len - 1] == ' ' || h[len - 1] == '\t')) --len;
+        long lo, hi;
+        if (*p == '[') {
+            p = skip_ws(p + 1);
+            bool ok = read_dec(&p, &lo);
+            p = skip_ws(p);
+            ok = ok && p[0] == '.' && p[1] == '.';
+            if (ok) p = skip_ws(p + 2);
+            ok = ok && read_dec(&p, &hi);
+            if (ok) p = skip_ws(p);
+            if (!ok || *p != ']') { set_error("%s: malformed range '%.40s'", path, line); return MDRL_EIO; }
+            ++p;
+        } else if (!read_dec(&p, &lo)) {
+            set_error("%s: malformed address '%.40s'", path, line);
+            return MDRL_EIO;
+        } else {
+            hi = lo;
+        }
+        p = skip_ws(p);
+        if (*p != ':') { set_error("%s: malformed line '%.40s'", path, line); return MDRL_EIO; }
+        const char *h = skip_ws(p + 1);
+        size_t len = 0;
+        while (hex_digit(h[len]) >= 0) ++len;
+        p = skip_ws(h + len);
+        if (*p != ';' || !is_blank_or_comment(p + 1)) { set_error("%s: malformed line '%.40s'", path, line); return MDRL_EIO; }
         const int np = w.width / 32;
         uint32_t v[3];
         if (len == 0 || !parse_hex_value(h, len, np, v)) { set_error("%s: malformed value '%.40s'", path, line); return MDRL_EIO; }
-        if (lo < 0 || hi < lo || hi >= w.depth) { set_error("%s: address %ld beyond DEPTH %d", path, hi, w.depth); return MDRL_EIO; }
-        for (long a = lo; a <= hi; ++a)
+        if (hi < lo || hi >= w.depth) { set_error("%s: address %ld beyond DEPTH %d", path, hi, w.depth); return MDRL_EIO; }
+        for (long a = lo; a <= hi; ++a) {
             for (int k = 0; k < np; ++k) w.parts[(size_t)a * np + k] = v[k];
+            w.given[(size_t)a] = 1;
+        }
     }
-    set_error("%s: no END", path);
-    return MDRL_EIO;
+    if (!ended) { set_error("%s: no END", path); return MDRL_EIO; }
+    return MDRL_OK;
 }
 
 int read_hex_words(FILE *f, const char *path, WordFile &w)
 {
     char line[1024];
     std::vector<uint32_t> parts;
+    std::vector<uint8_t> given;
     int width = 0, depth = 0;
     bool eof = false;
     while (fgets(line, sizeof line, f)) {
-        if (line[0] != ':') continue;
-        unsigned cnt, addr, type;
-        if (sscanf(line + 1, "%2x%4x%2x", &cnt, &addr, &type) != 3) { set_error("%s: malformed record", path); return MDRL_EIO; }
-        if (strlen(line) < 11 + 2 * (size_t)cnt) { set_error("%s: short record", path); return MDRL_EIO; }
-        unsigned sum = cnt + (addr >> 8) + (addr & 0xffu) + type, b[256], chk;
-        for (unsigned k = 0; k < cnt; ++k) {
-            if (sscanf(line + 9 + 2 * k, "%2x", &b[k]) != 1) { set_error("%s: short record", path); return MDRL_EIO; }
-            sum += b[k];
+        if (line[0] != ':') {
+            if (blank(line)) continue;
+            set_error("%s: line without record mark '%.40s'", path, line);
+            return MDRL_EIO;
         }
-        if (sscanf(line + 9 + 2 * cnt, "%2x", &chk) != 1 || ((sum + chk) & 0xffu) != 0) {
+        // ':' cc aaaa tt data[cc] ss, every field two hex digits a byte, then the line end
+        const size_t len = strlen(line);
+        const int cnt = len >= 3 ? hex_byte(line + 1) : -1;
+        if (cnt < 0 || len < 11 + 2 * (size_t)cnt) { set_error("%s: malformed record '%.40s'", path, line); return MDRL_EIO; }
+        int b[5 + 255];
+        unsigned sum = 0;
+        for (int k = 0; k < 5 + cnt; ++k) {
+            b[k] = hex_byte(line + 1 + 2 * k);
+            if (b[k] < 0) { set_error("%s: malformed record '%.40s'", path, line); return MDRL_EIO; }
+            sum += (unsigned)b[k];
+        }
+        if (!blank(line + 11 + 2 * cnt)) { set_error("%s: trailing characters in record '%.40s'", path, line); return MDRL_EIO; }
+        const unsigned addr = ((unsigned)b[1] << 8) | (unsigned)b[2], type = (unsigned)b[3];
+        const int *data = b + 4;
+        if ((sum & 0xffu) != 0) {
             set_error("%s: checksum mismatch at word %u", path, addr);
             return MDRL_EIO;
         }
         if (type == 1) { eof = true; break; }
         if (type != 0) continue;
-        if (cnt != 4 && cnt != 12) { set_error("%s: %u-byte record (4 or 12 expected)", path, cnt); return MDRL_EIO; }
-        if (width == 0) width = (int)cnt * 8;
-        if ((int)cnt * 8 != width) { set_error("%s: mixed record sizes", path); return MDRL_EIO; }
+        if (cnt != 4 && cnt != 12) { set_error("%s: %d-byte record (4 or 12 expected)", path, cnt); return MDRL_EIO; }
+        if (width == 0) width = cnt * 8;
+        if (cnt * 8 != width) { set_error("%s: mixed record sizes", path); return MDRL_EIO; }
         const int np = width / 32;
         if ((int)addr + 1 > depth) {
             depth = (int)addr + 1;
             parts.resize((size_t)depth * np, 0u);
+            given.resize((size_t)depth, 0);
         }
+        given[addr] = 1;
         for (int k = 0; k < np; ++k)
-            parts[(size_t)addr * np + k] = (b[4 * k] << 24) | (b[4 * k + 1] << 16) | (b[4 * k + 2] << 8) | b[4 * k + 3];
+            parts[(size_t)addr * np + k] = ((uint32_t)data[4 * k] << 24) | ((uint32_t)data[4 * k + 1] << 16) |
+                                           ((uint32_t)data[4 * k + 2] << 8) | (uint32_t)data[4 * k + 3];
     }
     if (!eof) { set_error("%s: no end-of-file record", path); return MDRL_EIO; }
     w.width = width;
     w.depth = depth;
     w.parts.swap(parts);
+    w.given.swap(given);
     return MDRL_OK;
 }
 
-int read_words(const char *path, WordFile &w)
+}  // namespace
+
+int mdrl::read_words(const char *path, WordFile &w)
 {
     FILE *f = fopen(path, "rb");
     if (!f) { set_error("cannot open %s", path); return MDRL_EIO; }
@@ -164,6 +292,10 @@ int read_words(const char *path, WordFile &w)
     fclose(f);
     return rc;
 }
+
+namespace {
+
+using mdrl::read_words;
 
 float as_float(uint32_t u)
 {

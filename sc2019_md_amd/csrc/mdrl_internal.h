@@ -27,6 +27,17 @@ int hip_fail(hipError_t e, const char *what);   // records the error, returns MD
         if (_e != hipSuccess) return ::mdrl::hip_fail(_e, #call);  \
     } while (0)
 
+// A memory image read from a Quartus .mif or an Intel HEX file (mdrl_cells.cpp): depth
+// words of `width` bits (32 or 96), each as width / 32 parts, most significant first;
+// given[a] says whether the file assigned word a.
+struct WordFile {
+    int width = 0;
+    int depth = 0;
+    std::vector<uint32_t> parts;
+    std::vector<uint8_t> given;
+};
+int read_words(const char *path, WordFile &w);   // MDRL_OK, or an error recorded
+
 // Fixed-point scale: forces accumulate as int64 multiples of 2^-32 kJ/mol/nm.
 constexpr double kFixScale = 4294967296.0;
 

@@ -7,6 +7,8 @@
 //         (O/HDL_Source/c0_14.hex)
 //   .txt  one decimal per line (M/c0_14.txt, written by
 //         M/LJ_no_smooth_poly_interpolation_function.m:248-258)
+// .mif and .hex go through the strict memory-image reader of mdrl_cells.cpp; a table must
+// assign every entry, and a damaged file is an error, never a shorter or absent table.
 #include <dlfcn.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -51,58 +53,20 @@ static float word_to_float(uint32_t u)
     return f;
 }
 
-static int read_mif(FILE *f, float *out, int cap)
-{
-    char line[512];
-    int n = 0, in_content = 0;
-    while (fgets(line, sizeof line, f)) {
-        if (!in_content) {
-            if (strncmp(line, "BEGIN", 5) == 0) in_content = 1;
-            continue;
-        }
-        if (strncmp(line, "END", 3) == 0) break;
-        unsigned addr;
-        char hex[64];
-        if (sscanf(line, " %u : %63[0-9A-Fa-f] ;", &addr, hex) != 2) continue;
-        if ((int)addr >= cap) return MDRL_EIO;
-        out[addr] = word_to_float((uint32_t)strtoul(hex, nullptr, 16));
-        if ((int)addr + 1 > n) n = (int)addr + 1;
-    }
-    return n;
-}
-
-static int read_ihex(FILE *f, float *out, int cap)
+// One decimal per line; blank lines are skipped, anything else is an error.
+static int read_txt(FILE *f, const char *path, float *out, int cap)
 {
     char line[512];
     int n = 0;
     while (fgets(line, sizeof line, f)) {
-        if (line[0] != ':') continue;
-        unsigned cnt, addr, type;
-        if (sscanf(line + 1, "%2x%4x%2x", &cnt, &addr, &type) != 3) return MDRL_EIO;
-        if (type == 1) break;
-        if (type != 0 || cnt != 4) continue;
-        unsigned b[4], sum = cnt + (addr >> 8) + (addr & 0xff) + type, chk;
-        for (int k = 0; k < 4; ++k) {
-            if (sscanf(line + 9 + 2 * k, "%2x", &b[k]) != 1) return MDRL_EIO;
-            sum += b[k];
-        }
-        if (sscanf(line + 17, "%2x", &chk) != 1 || ((sum + chk) & 0xff) != 0) return MDRL_EIO;
-        if ((int)addr >= cap) return MDRL_EIO;
-        out[addr] = word_to_float((b[0] << 24) | (b[1] << 16) | (b[2] << 8) | b[3]);
-        if ((int)addr + 1 > n) n = (int)addr + 1;
-    }
-    return n;
-}
-
-static int read_txt(FILE *f, float *out, int cap)
-{
-    char line[512];
-    int n = 0;
-    while (fgets(line, sizeof line, f)) {
+        const char *p = line;
+        while (*p == ' ' || *p == '\t') ++p;
+        if (*p == 0 || *p == '\r' || *p == '\n') continue;
         char *end;
-        double v = strtod(line, &end);
-        if (end == line) continue;
-        if (n >= cap) return MDRL_EIO;
+        const double v = strtod(p, &end);
+        while (end != p && (*end == ' ' || *end == '\t' || *end == '\r' || *end == '\n')) ++end;
+        if (end == p || *end != 0) { set_error("%s: line %d is not a number", path, n + 1); return MDRL_EIO; }
+        if (n >= cap) { set_error("%s: more than %d entries", path, cap); return MDRL_EIO; }
         out[n++] = (float)v;
     }
     return n;
@@ -111,15 +75,23 @@ static int read_txt(FILE *f, float *out, int cap)
 extern "C" int md_rl_table_read_file(const char *path, float *out, int cap)
 {
     if (!path || !out || cap <= 0) { set_error("md_rl_table_read_file: bad argument"); return MDRL_EINVAL; }
+    const char *dot = strrchr(path, '.');
+    if (dot && (strcmp(dot, ".mif") == 0 || strcmp(dot, ".hex") == 0)) {
+        mdrl::WordFile w;
+        const int rc = mdrl::read_words(path, w);
+        if (rc) return rc;
+        if (w.width != 32) { set_error("%s: %d-bit words, a table has 32", path, w.width); return MDRL_EIO; }
+        if (w.depth > cap) { set_error("%s: %d entries, room for %d", path, w.depth, cap); return MDRL_EIO; }
+        for (int a = 0; a < w.depth; ++a) {
+            if (!w.given[(size_t)a]) { set_error("%s: entry %d missing", path, a); return MDRL_EIO; }
+            out[a] = word_to_float(w.parts[(size_t)a]);
+        }
+        return w.depth;
+    }
     FILE *f = fopen(path, "r");
     if (!f) { set_error("cannot open %s", path); return MDRL_EIO; }
-    const char *dot = strrchr(path, '.');
-    int n;
-    if (dot && strcmp(dot, ".mif") == 0) n = read_mif(f, out, cap);
-    else if (dot && strcmp(dot, ".hex") == 0) n = read_ihex(f, out, cap);
-    else n = read_txt(f, out, cap);
+    const int n = read_txt(f, path, out, cap);
     fclose(f);
-    if (n < 0) set_error("malformed table file %s", path);
     return n;
 }
 
@@ -173,6 +145,7 @@ extern "C" int md_rl_table_load(md_rl_table **out, const char *dir, int exp_offs
                 if (!f) continue;
                 fclose(f);
                 got = md_rl_table_read_file(path, b.data(), nent);
+                if (got < 0) return got;            // present but unreadable: not an absent term
                 break;
             }
             if (got < 0) { b.clear(); break; }

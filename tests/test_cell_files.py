@@ -210,3 +210,88 @@ def test_demo64_forces_bit_exact(demo64):
     want, npairs = rl.forces_fix()
     assert st.pair_evals == npairs
     assert np.array_equal(got, want)
+
+
+def flips(src):
+    """Every single-byte corruption (XOR 0x5a) of a file's bytes."""
+    for i in range(len(src)):
+        b = bytearray(src)
+        b[i] ^= 0x5a
+        yield i, bytes(b)
+
+
+def accepts(read, path, data):
+    with open(path, "wb") as f:
+        f.write(data)
+    try:
+        return read(path)
+    except mdrl.MdrlError:
+        return None
+
+
+def test_damaged_hex_files_rejected(meminit, tmp_path):
+    """Intel HEX carries a checksum per record: every single-byte corruption of a cell file
+    and of a table file, and every truncation before the end-of-file record, is an error
+    (not a different memory)."""
+    cases = [(os.path.join(meminit, "cell_ini_file_3_2_3.hex"), mdrl.read_cell_file, 1),
+             (os.path.join(GOLDEN, "c0_14.hex"), mdrl.read_table_file, 13)]
+    for path, read, stride in cases:
+        src = open(path, "rb").read()
+        tmp = str(tmp_path / ("damaged" + path[-4:]))
+        ok = [i for i, b in flips(src) if i % stride == 0 and accepts(read, tmp, b) is not None]
+        assert ok == [], (path, ok[:5])
+        end = src.rindex(b":00000001FF") + 11
+        ok = [c for c in range(0, end, stride) if accepts(read, tmp, src[:c]) is not None]
+        assert ok == [], (path, ok[:5])
+
+
+def test_damaged_mif_files_rejected(meminit, tmp_path):
+    """A .mif has no checksum: a corruption that turns a value digit into another hex digit
+    reads as that one changed word, and every other single-byte corruption and every
+    truncation before "END;" is an error."""
+    path = os.path.join(meminit, "cell_ini_file_3_2_3.mif")
+    src = open(path, "rb").read()
+    want = mdrl.read_cell_file(path)
+    tmp = str(tmp_path / "damaged.mif")
+    hexdigits = b"0123456789ABCDEFabcdef"
+    for i, b in flips(src):
+        got = accepts(mdrl.read_cell_file, tmp, b)
+        if got is None:
+            continue
+        line_start = src.rindex(b"\n", 0, i) + 1
+        in_value = b":" in src[line_start:i] and b";" in src[i:src.index(b"\n", i)]
+        assert in_value and src[i] in hexdigits and b[i] in hexdigits, (i, src[i:i + 1])
+        assert got.shape == want.shape and np.count_nonzero(got.view(np.uint32) != want.view(np.uint32)) <= 1, i
+    end = src.rindex(b"END;") + 4
+    ok = [c for c in range(end) if accepts(mdrl.read_cell_file, tmp, src[:c]) is not None]
+    assert ok == [], ok[:5]
+    tab = os.path.join(os.path.dirname(mdrl.__file__), "tables", "apoa1", "c0_3.mif")
+    src = open(tab, "rb").read()
+    assert accepts(mdrl.read_table_file, tmp, src).shape == (3584,)
+    lines = src.split(b"\n")
+    assert lines[100].startswith(b"94 : ")
+    semi = src.index(b";", src.index(b"\n0 : "))
+    for bad in (b"\n".join(lines[:100] + lines[101:]),                        # entry 94 missing
+                src.replace(b"DATA_RADIX = HEX", b"DATA_RADIX = DEC", 1),   # radix the reader lacks
+                src.replace(b"BEGIN", b"BEGINS", 1),
+                src[:semi] + src[semi + 1:],                                 # word 0 without ';'
+                src.replace(b"\n1 : ", b"\n1x : ", 1)):
+        assert accepts(mdrl.read_table_file, tmp, bad) is None
+
+
+def test_unreadable_table_term_is_an_error(tmp_path):
+    """A table directory whose r^-3 file is present but damaged fails to load; only a
+    missing file leaves a term out."""
+    src = os.path.join(os.path.dirname(mdrl.__file__), "tables", "apoa1")
+    for name in os.listdir(src):
+        data = open(os.path.join(src, name), "rb").read()
+        if name == "c0_3.mif":
+            data = data.replace(b"5 : ", b"5 ; ", 1)
+        (tmp_path / name).write_bytes(data)
+    with pytest.raises(mdrl.MdrlError) as e:
+        mdrl.Table.load(str(tmp_path))
+    assert b"c0_3.mif" in mdrl.lib().md_rl_last_error() or "c0_3" in str(e.value)
+    for k in (0, 1):
+        os.remove(str(tmp_path / ("c%d_3.mif" % k)))
+    t = mdrl.Table.load(str(tmp_path))
+    assert t.get(2)[0] is None and t.get(0)[0] is not None
