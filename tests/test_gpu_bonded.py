@@ -57,6 +57,23 @@ def test_chain_forces_bit_exact(apoa1, index):
     assert np.all(got.sum(axis=0) == 0)
 
 
+def test_chain_forces_every_schedule(apoa1):
+    """Exclusion windows under every force-kernel schedule (including the home-set
+    culling of 16 and 32 atoms per wave): int64 forces and pair counts equal the oracle's."""
+    s = mdrl.SyntheticSystem(5, 0)
+    want, wpairs, _, _ = oracle_forces(s, apoa1)
+    g = mdrl.System.from_synthetic(s)
+    for sc in [(a, t, gr, 1) for a in (16, 32, 64) for t in (128, 256) for gr in (1, 4)]:
+        say("chains: schedule", sc)
+        g.set_schedule(*sc)
+        g.force_eval()
+        got = g.forces_fixed()
+        assert g.stats().pair_evals == wpairs, sc
+        bad = np.nonzero(np.any(got != want, axis=1))[0]
+        assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
+    g.close()
+
+
 def test_chain_steps_bit_exact(apoa1):
     s = mdrl.SyntheticSystem(5, 2)
     nsteps = 5
