@@ -16,9 +16,253 @@ namespace mdrl {
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 // ---------------------------------------------------------------------------------
+// Bonded terms (Bonded/Bonded_Top.v:202-380 streams bonds, angles and torsions through
+// calc_bond_v2.v, calc_angle_v2.v and calcTorsion_v2.v): one thread per term, FP32 forces
+// in kJ/mol/nm (the constants carry the factor 10), added into the same fixed-point
+// buffer as the range-limited forces.  Terms name atoms by input index; slot_of maps
+// them to the current slots.  The arithmetic is written operation by operation (the
+// build does not contract), so the oracle restates it bit for bit.
+// ---------------------------------------------------------------------------------
+
+struct BondedArgs {
+    const float4 *pos;
+    const int *slot_of;
+    const int *slot_sys;
+    const SysGeom *geom;
+    const int4 *atoms;
+    const float4 *params;
+    int nterms;
+    long long *fix;
+    float *f32;
+    double *term_e;
+    unsigned long long *counters;
+    const int4 *blocks;       // per block: first term, end term, first atom, atom span
+    int nblocks;
+};
+
+__device__ __forceinline__ float img_f(float d, float L)
+{
+    const float h = 0.5f * L;
+    if (d > h) return d - L;
+    if (d < -h) return d + L;
+    return d;
+}
+__device__ __forceinline__ float dot3f(const float *a, const float *b)
+{
+    float r = a[0] * b[0] + a[1] * b[1];
+    return r + a[2] * b[2];
+}
+__device__ __forceinline__ void cross3f(const float *a, const float *b, float *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// atan2 for the bond angle: reduce to a = min/max in [0, 1]; above tan(pi/12) use
+// atan a = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3)); then the Taylor series of atan to
+// t^11 (|t| <= 0.268, next term < 3e-9).
+__device__ __forceinline__ float angle_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    const float a = mn / mx;
+    float t = a, off = 0.0f;
+    if (a > 0.26794919f) {
+        const float num = a * 1.7320508f - 1.0f;
+        const float den = a + 1.7320508f;
+        t = num / den;
+        off = 0.52359878f;
+    }
+    const float t2 = t * t;
+    float p = -0.09090909f;
+    p = p * t2 + 0.11111111f;
+    p = p * t2 - 0.14285715f;
+    p = p * t2 + 0.2f;
+    p = p * t2 - 0.33333334f;
+    p = p * t2 + 1.0f;
+    float r = p * t;
+    r = off + r;
+    if (ay > ax) r = 1.5707964f - r;
+    if (x < 0.0f) r = 3.1415927f - r;
+    return y < 0.0f ? -r : r;
+}
+
+// The forces of a term's atoms, added atomically.  In fixed point the last atom takes
+// minus the sum of the others' converted forces, so each term's forces sum to zero exactly.
+// Forces go to a sink: the LDS accumulators of the term block (atom index - first atom of
+// the block), flushed to HBM once per atom by k_bonded or by the bonded tasks of
+// k_rl_halfshell.  The terms of a block touch at most kBondedBlockSpan consecutive input
+// indices.
+struct BondedSink {
+    unsigned long long *acc;   // 3 per atom of the block
+    float *accf;               // FP32-atomics mode: 3 per atom
+    int lo;                    // first input index of the block
+};
+
+template <int NA>
+__device__ __forceinline__ void bonded_add(const BondedSink &k_, const int *atom, const float (*f)[3],
+                                           unsigned &flag)
+{
+    if (k_.accf) {
+        for (int k = 0; k < NA; ++k)
+            for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.accf[3 * (atom[k] - k_.lo) + d], f[k][d]);
+        return;
+    }
+    unsigned long long sum[3] = {0ull, 0ull, 0ull};
+    for (int k = 0; k < NA; ++k) {
+        long long q[3];
+        if (k < NA - 1) {
+            fix3(f[k][0], f[k][1], f[k][2], q[0], q[1], q[2], flag);
+            for (int d = 0; d < 3; ++d) sum[d] += (unsigned long long)q[d];
+        } else {
+            for (int d = 0; d < 3; ++d) q[d] = (long long)(0ull - sum[d]);
+        }
+        for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.acc[3 * (atom[k] - k_.lo) + d], (unsigned long long)q[d]);
+    }
+}
+
+// One term.  ENERGY: return the FP64 energy of the term (from the same FP32
+// intermediates) instead of adding forces.
+template <bool ENERGY>
+__device__ __forceinline__ double bonded_term(const BondedArgs &a, int t, const BondedSink &sink,
+                                              unsigned &flag)
+{
+    const int4 at = a.atoms[t];
+    const float4 pp = a.params[t];
+    const int s0 = a.slot_of[at.x], s1 = a.slot_of[at.y];
+    const SysGeom &G = a.geom[a.slot_sys[s0]];
+    const float4 p1 = a.pos[s0], p2 = a.pos[s1];
+    const float x1[3] = {p1.x, p1.y, p1.z}, x2[3] = {p2.x, p2.y, p2.z};
+    double e = 0.0;
+    if (at.z < 0) {
+        // bond, Bonded/calc_bond_v2.v:83-110
+        float r12[3];
+        for (int d = 0; d < 3; ++d) r12[d] = img_f(x1[d] - x2[d], G.box[d]);
+        const float r = sqrtf(dot3f(r12, r12));
+        const float dr = r - pp.x;
+        if (ENERGY) {
+            const double ddr = (double)r - (double)pp.x;
+            e = (double)pp.y / 10.0 * ddr * ddr;
+        } else {
+            float dp = 2.0f * pp.y;
+            dp = dp * dr;
+            const float c = -dp / r;
+            float f[2][3];
+            for (int d = 0; d < 3; ++d) { f[0][d] = c * r12[d]; f[1][d] = -f[0][d]; }
+            const int ids[2] = {at.x, at.y};
+            bonded_add<2>(sink, ids, f, flag);
+        }
+    } else if (at.w < 0) {
+        // angle, Bonded/calc_angle_v2.v:113-147 (no Urey-Bradley term)
+        const int s2 = a.slot_of[at.z];
+        const float4 p3 = a.pos[s2];
+        const float x3[3] = {p3.x, p3.y, p3.z};
+        float r12[3], r32[3], cr[3];
+        for (int d = 0; d < 3; ++d) {
+            r12[d] = img_f(x1[d] - x2[d], G.box[d]);
+            r32[d] = img_f(x3[d] - x2[d], G.box[d]);
+        }
+        const float d12 = sqrtf(dot3f(r12, r12)), d32 = sqrtf(dot3f(r32, r32));
+        cross3f(r12, r32, cr);
+        const float y = sqrtf(dot3f(cr, cr)), xx = dot3f(r12, r32);
+        const float th = angle_atan2f(y, xx);
+        if (ENERGY) {
+            const double dth = (double)th - (double)pp.x;
+            e = (double)pp.y / 10.0 * dth * dth;
+        } else {
+            const float dd = d12 * d32;
+            const float cs = xx / dd, sn = y / dd;
+            if (sn > 1e-6f) {
+                float dp = 2.0f * pp.y;
+                dp = dp * (th - pp.x);
+                const float c = -dp;
+                const float a1 = sn * d12, a3 = sn * d32;
+                // order: end 1, end 3, then the vertex (minus their sum)
+                float f[3][3];
+                for (int d = 0; d < 3; ++d) {
+                    const float u12 = r12[d] / d12, u32 = r32[d] / d32;
+                    float g1 = u12 * cs - u32;
+                    g1 = g1 / a1;
+                    float g3 = u32 * cs - u12;
+                    g3 = g3 / a3;
+                    f[0][d] = c * g1;
+                    f[1][d] = c * g3;
+                    f[2][d] = -f[0][d] - f[1][d];
+                }
+                const int ids[3] = {at.x, at.z, at.y};
+                bonded_add<3>(sink, ids, f, flag);
+            }
+        }
+    } else {
+        // torsion, Bonded/calcTorsion_v2.v:70-274: k (1 + cos(n phi + d)) with the
+        // projection-form forces (no division by sin phi)
+        const int s2 = a.slot_of[at.z], s3 = a.slot_of[at.w];
+        const float4 p3 = a.pos[s2], p4 = a.pos[s3];
+        const float x3[3] = {p3.x, p3.y, p3.z}, x4[3] = {p4.x, p4.y, p4.z};
+        float rij[3], rkj[3], rkl[3], m[3], v[3];
+        for (int d = 0; d < 3; ++d) {
+            rij[d] = img_f(x1[d] - x2[d], G.box[d]);
+            rkj[d] = img_f(x3[d] - x2[d], G.box[d]);
+            rkl[d] = img_f(x3[d] - x4[d], G.box[d]);
+        }
+        cross3f(rij, rkj, m);
+        cross3f(rkj, rkl, v);
+        const float m2 = dot3f(m, m), v2 = dot3f(v, v);
+        if (m2 > 1e-12f && v2 > 1e-12f) {
+            const float kj2 = dot3f(rkj, rkj);
+            const float kj = sqrtf(kj2);
+            const float mv = sqrtf(m2) * sqrtf(v2);
+            const float cp = dot3f(m, v) / mv;
+            float sp = kj * dot3f(rij, v);
+            sp = sp / mv;
+            const int nper = (int)pp.y;
+            float cn = cp, snn = sp;
+            for (int k = 1; k < nper; ++k) {
+                const float c2 = cn * cp - snn * sp;
+                const float s2n = snn * cp + cn * sp;
+                cn = c2;
+                snn = s2n;
+            }
+            const float sarg = snn * pp.z + cn * pp.w;
+            const float carg = cn * pp.z - snn * pp.w;
+            if (ENERGY) {
+                e = (double)pp.x / 10.0 * (1.0 + (double)carg);
+            } else {
+                float ddphi = -pp.x * pp.y;
+                ddphi = ddphi * sarg;
+                float ca = -ddphi * kj;
+                ca = ca / m2;
+                float cb = ddphi * kj;
+                cb = cb / v2;
+                const float pq = dot3f(rij, rkj) / kj2, qq = dot3f(rkl, rkj) / kj2;
+                // order: i, j, l, then k (minus their sum)
+                float f[4][3];
+                for (int d = 0; d < 3; ++d) {
+                    const float fi = ca * m[d], fl = cb * v[d];
+                    const float sv = pq * fi - qq * fl;
+                    f[0][d] = fi;
+                    f[1][d] = -(fi - sv);
+                    f[2][d] = fl;
+                    f[3][d] = -(fl + sv);
+                }
+                const int ids[4] = {at.x, at.y, at.w, at.z};
+                bonded_add<4>(sink, ids, f, flag);
+            }
+        }
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------------
 // k_rl_halfshell: one workgroup of 12 waves per task = (home cell, group of half-shell
 // offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
-// first) from a counter.
+// first) from a counter.  With exclusion windows (bonded systems) the blocks of bonded
+// terms are tasks of the same launch, after the range-limited ones, so the two force
+// kinds share one persistent grid and the short bonded blocks fill its tail
+// (Bonded/Bonded_Top.v:202-380 runs beside the pair pipelines;
+// Summation_Logic/Force_Summation_Unit.v:62-126 adds the results).
 //
 // The home cell's atoms are staged in LDS (tile of TI = 256) and each neighbour cell of
 // the group (pre-shifted by its periodic image) as a tile of TJ.  A lane holds one home
@@ -69,6 +313,7 @@ struct ForceArgs {
     int gstart[8];
     const int *orig;          // slot -> input index (exclusions)
     const uint32_t *excl;     // exclusion windows by input index, or NULL
+    BondedArgs bd;            // bonded term blocks, tasks ntasks .. ntasks + bd.nblocks - 1
 };
 
 // EXC: nonbonded exclusions; a queued pair whose neighbour lies in the home atom's
@@ -79,6 +324,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     constexpr int STR = COUL ? 6 : 4;
     constexpr int P = 64 / APW;
     static_assert(TJ <= 256 && TJ <= NTHR, "queue entries hold 8-bit neighbour indices");
+    static_assert(kBondedBlockSpan <= TI, "a bonded block accumulates in the home-tile accumulators");
     extern __shared__ float s_tab[];
     __shared__ float4 s_i[TI];
     __shared__ float4 s_j[TJ];
@@ -119,6 +365,32 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     unsigned long long npairs = 0, ncand = 0, ndrop = 0;
     const float rc2 = a.rc2;
 
+    // One block of bonded terms: forces accumulate in the home-tile accumulators
+    // (3 x kBondedBlockSpan words: fixed point, or FP32 in FP32-atomics mode) and reach HBM
+    // with one atomic per touched atom and component.
+    auto bonded_task = [&](int b) {
+        const int4 blk = a.bd.blocks[b];   // first term, end term, first atom, atom span
+        for (int k = tid; k < 3 * kBondedBlockSpan; k += NTHR) s_iacc[k] = 0ull;
+        lds_barrier();
+        const BondedSink sink{F32 ? nullptr : s_iacc, F32 ? s_iacc_f : nullptr, blk.z};
+        for (int t = blk.x + tid; t < blk.y; t += NTHR) bonded_term<false>(a.bd, t, sink, flag);
+        lds_barrier();
+        for (int k = tid; k < 3 * blk.w; k += NTHR) {
+            const int slot = a.bd.slot_of[blk.z + k / 3];
+            if (F32) {
+                const float v = s_iacc_f[k];
+                if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)slot + k % 3], v);
+            } else {
+                const unsigned long long v = s_iacc[k];
+                if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot + k % 3), v);
+            }
+        }
+    };
+    // bonded tasks are compiled into the exclusion variants only (the others keep their
+    // registers); the launcher runs k_bonded beside the others
+    constexpr bool BND = EXC;
+    const int ntot = a.ntasks + (BND ? a.bd.nblocks : 0);
+
     for (int it = 0;; ++it) {
         int task;
         if (a.tasks) {
@@ -127,7 +399,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             // tasks c + kQueues * k, k = (workgroups of the queue) + its counter
             int t = (int)blockIdx.x;
             if (it > 0) {
-                if ((int)gridDim.x >= a.ntasks) break;
+                if ((int)gridDim.x >= ntot) break;
                 const int c = (int)blockIdx.x % kQueues;
                 const int nb = ((int)gridDim.x - c + kQueues - 1) / kQueues;
                 lds_barrier();
@@ -136,10 +408,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 lds_barrier();
                 t = s_task;
             }
-            if (t >= a.ntasks) break;
+            if (t >= ntot) break;
+            if (BND && t >= a.ntasks) { bonded_task(t - a.ntasks); continue; }
             task = a.tasks[t];
         } else {
             if (it > 0) break;
+            if (BND && (int)blockIdx.x >= a.ntasks) { bonded_task((int)blockIdx.x - a.ntasks); continue; }
             task = blockIdx.x;
         }
         const int h = task / a.ngroups, grp = task - h * a.ngroups;
@@ -413,244 +687,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---------------------------------------------------------------------------------
-// Bonded terms (Bonded/Bonded_Top.v:202-380 streams bonds, angles and torsions through
-// calc_bond_v2.v, calc_angle_v2.v and calcTorsion_v2.v): one thread per term, FP32 forces
-// in kJ/mol/nm (the constants carry the factor 10), added into the same fixed-point
-// buffer as the range-limited forces.  Terms name atoms by input index; slot_of maps
-// them to the current slots.  The arithmetic is written operation by operation (the
-// build does not contract), so the oracle restates it bit for bit.
-// ---------------------------------------------------------------------------------
-
-struct BondedArgs {
-    const float4 *pos;
-    const int *slot_of;
-    const int *slot_sys;
-    const SysGeom *geom;
-    const int4 *atoms;
-    const float4 *params;
-    int nterms;
-    long long *fix;
-    float *f32;
-    double *term_e;
-    unsigned long long *counters;
-    const int4 *blocks;       // per block: first term, end term, first atom, atom span
-    int nblocks;
-};
-
-__device__ __forceinline__ float img_f(float d, float L)
-{
-    const float h = 0.5f * L;
-    if (d > h) return d - L;
-    if (d < -h) return d + L;
-    return d;
-}
-__device__ __forceinline__ float dot3f(const float *a, const float *b)
-{
-    float r = a[0] * b[0] + a[1] * b[1];
-    return r + a[2] * b[2];
-}
-__device__ __forceinline__ void cross3f(const float *a, const float *b, float *c)
-{
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
-}
-
-// atan2 for the bond angle: reduce to a = min/max in [0, 1]; above tan(pi/12) use
-// atan a = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3)); then the Taylor series of atan to
-// t^11 (|t| <= 0.268, next term < 3e-9).
-__device__ __forceinline__ float angle_atan2f(float y, float x)
-{
-    const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
-    if (mx == 0.0f) return 0.0f;
-    const float a = mn / mx;
-    float t = a, off = 0.0f;
-    if (a > 0.26794919f) {
-        const float num = a * 1.7320508f - 1.0f;
-        const float den = a + 1.7320508f;
-        t = num / den;
-        off = 0.52359878f;
-    }
-    const float t2 = t * t;
-    float p = -0.09090909f;
-    p = p * t2 + 0.11111111f;
-    p = p * t2 - 0.14285715f;
-    p = p * t2 + 0.2f;
-    p = p * t2 - 0.33333334f;
-    p = p * t2 + 1.0f;
-    float r = p * t;
-    r = off + r;
-    if (ay > ax) r = 1.5707964f - r;
-    if (x < 0.0f) r = 3.1415927f - r;
-    return y < 0.0f ? -r : r;
-}
-
-// The forces of a term's atoms, added atomically.  In fixed point the last atom takes
-// minus the sum of the others' converted forces, so each term's forces sum to zero exactly.
-// Forces go to a sink: the LDS accumulators of the term block (atom index - first atom of
-// the block), flushed to HBM once per atom by k_bonded.
-struct BondedSink {
-    unsigned long long *acc;   // 3 per atom of the block
-    float *accf;               // FP32-atomics mode: 3 per atom
-    int lo;                    // first input index of the block
-};
-
-template <int NA>
-__device__ __forceinline__ void bonded_add(const BondedSink &k_, const int *atom, const float (*f)[3],
-                                           unsigned &flag)
-{
-    if (k_.accf) {
-        for (int k = 0; k < NA; ++k)
-            for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.accf[3 * (atom[k] - k_.lo) + d], f[k][d]);
-        return;
-    }
-    unsigned long long sum[3] = {0ull, 0ull, 0ull};
-    for (int k = 0; k < NA; ++k) {
-        long long q[3];
-        if (k < NA - 1) {
-            fix3(f[k][0], f[k][1], f[k][2], q[0], q[1], q[2], flag);
-            for (int d = 0; d < 3; ++d) sum[d] += (unsigned long long)q[d];
-        } else {
-            for (int d = 0; d < 3; ++d) q[d] = (long long)(0ull - sum[d]);
-        }
-        for (int d = 0; d < 3; ++d) atomic_add_wg(&k_.acc[3 * (atom[k] - k_.lo) + d], (unsigned long long)q[d]);
-    }
-}
-
-// One term.  ENERGY: return the FP64 energy of the term (from the same FP32
-// intermediates) instead of adding forces.
-template <bool ENERGY>
-__device__ __forceinline__ double bonded_term(const BondedArgs &a, int t, const BondedSink &sink,
-                                              unsigned &flag)
-{
-    const int4 at = a.atoms[t];
-    const float4 pp = a.params[t];
-    const int s0 = a.slot_of[at.x], s1 = a.slot_of[at.y];
-    const SysGeom &G = a.geom[a.slot_sys[s0]];
-    const float4 p1 = a.pos[s0], p2 = a.pos[s1];
-    const float x1[3] = {p1.x, p1.y, p1.z}, x2[3] = {p2.x, p2.y, p2.z};
-    double e = 0.0;
-    if (at.z < 0) {
-        // bond, Bonded/calc_bond_v2.v:83-110
-        float r12[3];
-        for (int d = 0; d < 3; ++d) r12[d] = img_f(x1[d] - x2[d], G.box[d]);
-        const float r = sqrtf(dot3f(r12, r12));
-        const float dr = r - pp.x;
-        if (ENERGY) {
-            const double ddr = (double)r - (double)pp.x;
-            e = (double)pp.y / 10.0 * ddr * ddr;
-        } else {
-            float dp = 2.0f * pp.y;
-            dp = dp * dr;
-            const float c = -dp / r;
-            float f[2][3];
-            for (int d = 0; d < 3; ++d) { f[0][d] = c * r12[d]; f[1][d] = -f[0][d]; }
-            const int ids[2] = {at.x, at.y};
-            bonded_add<2>(sink, ids, f, flag);
-        }
-    } else if (at.w < 0) {
-        // angle, Bonded/calc_angle_v2.v:113-147 (no Urey-Bradley term)
-        const int s2 = a.slot_of[at.z];
-        const float4 p3 = a.pos[s2];
-        const float x3[3] = {p3.x, p3.y, p3.z};
-        float r12[3], r32[3], cr[3];
-        for (int d = 0; d < 3; ++d) {
-            r12[d] = img_f(x1[d] - x2[d], G.box[d]);
-            r32[d] = img_f(x3[d] - x2[d], G.box[d]);
-        }
-        const float d12 = sqrtf(dot3f(r12, r12)), d32 = sqrtf(dot3f(r32, r32));
-        cross3f(r12, r32, cr);
-        const float y = sqrtf(dot3f(cr, cr)), xx = dot3f(r12, r32);
-        const float th = angle_atan2f(y, xx);
-        if (ENERGY) {
-            const double dth = (double)th - (double)pp.x;
-            e = (double)pp.y / 10.0 * dth * dth;
-        } else {
-            const float dd = d12 * d32;
-            const float cs = xx / dd, sn = y / dd;
-            if (sn > 1e-6f) {
-                float dp = 2.0f * pp.y;
-                dp = dp * (th - pp.x);
-                const float c = -dp;
-                const float a1 = sn * d12, a3 = sn * d32;
-                // order: end 1, end 3, then the vertex (minus their sum)
-                float f[3][3];
-                for (int d = 0; d < 3; ++d) {
-                    const float u12 = r12[d] / d12, u32 = r32[d] / d32;
-                    float g1 = u12 * cs - u32;
-                    g1 = g1 / a1;
-                    float g3 = u32 * cs - u12;
-                    g3 = g3 / a3;
-                    f[0][d] = c * g1;
-                    f[1][d] = c * g3;
-                    f[2][d] = -f[0][d] - f[1][d];
-                }
-                const int ids[3] = {at.x, at.z, at.y};
-                bonded_add<3>(sink, ids, f, flag);
-            }
-        }
-    } else {
-        // torsion, Bonded/calcTorsion_v2.v:70-274: k (1 + cos(n phi + d)) with the
-        // projection-form forces (no division by sin phi)
-        const int s2 = a.slot_of[at.z], s3 = a.slot_of[at.w];
-        const float4 p3 = a.pos[s2], p4 = a.pos[s3];
-        const float x3[3] = {p3.x, p3.y, p3.z}, x4[3] = {p4.x, p4.y, p4.z};
-        float rij[3], rkj[3], rkl[3], m[3], v[3];
-        for (int d = 0; d < 3; ++d) {
-            rij[d] = img_f(x1[d] - x2[d], G.box[d]);
-            rkj[d] = img_f(x3[d] - x2[d], G.box[d]);
-            rkl[d] = img_f(x3[d] - x4[d], G.box[d]);
-        }
-        cross3f(rij, rkj, m);
-        cross3f(rkj, rkl, v);
-        const float m2 = dot3f(m, m), v2 = dot3f(v, v);
-        if (m2 > 1e-12f && v2 > 1e-12f) {
-            const float kj2 = dot3f(rkj, rkj);
-            const float kj = sqrtf(kj2);
-            const float mv = sqrtf(m2) * sqrtf(v2);
-            const float cp = dot3f(m, v) / mv;
-            float sp = kj * dot3f(rij, v);
-            sp = sp / mv;
-            const int nper = (int)pp.y;
-            float cn = cp, snn = sp;
-            for (int k = 1; k < nper; ++k) {
-                const float c2 = cn * cp - snn * sp;
-                const float s2n = snn * cp + cn * sp;
-                cn = c2;
-                snn = s2n;
-            }
-            const float sarg = snn * pp.z + cn * pp.w;
-            const float carg = cn * pp.z - snn * pp.w;
-            if (ENERGY) {
-                e = (double)pp.x / 10.0 * (1.0 + (double)carg);
-            } else {
-                float ddphi = -pp.x * pp.y;
-                ddphi = ddphi * sarg;
-                float ca = -ddphi * kj;
-                ca = ca / m2;
-                float cb = ddphi * kj;
-                cb = cb / v2;
-                const float pq = dot3f(rij, rkj) / kj2, qq = dot3f(rkl, rkj) / kj2;
-                // order: i, j, l, then k (minus their sum)
-                float f[4][3];
-                for (int d = 0; d < 3; ++d) {
-                    const float fi = ca * m[d], fl = cb * v[d];
-                    const float sv = pq * fi - qq * fl;
-                    f[0][d] = fi;
-                    f[1][d] = -(fi - sv);
-                    f[2][d] = fl;
-                    f[3][d] = -(fl + sv);
-                }
-                const int ids[4] = {at.x, at.y, at.w, at.z};
-                bonded_add<4>(sink, ids, f, flag);
-            }
-        }
-    }
-    return e;
-}
-
 // FP64 energy of every term (thread per term), for the fixed-order per-system reduction.
 __global__ __launch_bounds__(256) void k_bonded_energy(BondedArgs a)
 {
@@ -660,9 +696,11 @@ __global__ __launch_bounds__(256) void k_bonded_energy(BondedArgs a)
     a.term_e[t] = bonded_term<true>(a, t, BondedSink{nullptr, nullptr, 0}, flag);
 }
 
-// Forces of a block of terms (one workgroup): the terms of a block touch at most
-// kBondedBlockSpan consecutive input indices, so they accumulate in LDS (fixed point, or
-// FP32 in FP32-atomics mode) and reach HBM with one atomic per touched atom and component.
+// Forces of a block of terms (one workgroup), for the launches whose force kernel does not
+// take bonded tasks (no exclusion windows, FP32 atomics, parity or analytic mode): the
+// terms of a block touch at most kBondedBlockSpan consecutive input indices, so they
+// accumulate in LDS (fixed point, or FP32 in FP32-atomics mode) and reach HBM with one
+// atomic per touched atom and component.
 __global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
 {
     __shared__ unsigned long long s_acc[3 * kBondedBlockSpan];
@@ -742,9 +780,15 @@ static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, si
 int launch_force(DeviceSystem &s, hipStream_t st)
 {
     if (s.n == 0) return MDRL_OK;
-    if (s.nblocks > 0)
+    const bool f32 = s.flags & MDRL_ACC_FP32, par = s.flags & MDRL_EVAL_PARITY,
+               ana = s.flags & MDRL_EVAL_ANALYTIC, coul = s.coul;
+    // bonded blocks: the last tasks of the exclusion variants, else a kernel of their own
+    const bool fused = s.nblocks > 0 && s.excl && !f32 && !par && !ana;
+    if (s.nblocks > 0 && !fused)
         hipLaunchKernelGGL(k_bonded, dim3(s.nblocks), dim3(256), 0, st, bonded_args(s));
     ForceArgs a;
+    a.bd = bonded_args(s);
+    if (!fused) a.bd.nblocks = 0;
     a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
     a.geom = s.geom; a.cell_sys = s.cell_sys;
     a.tasks = s.sched.persistent ? s.tasks : nullptr;
@@ -757,11 +801,10 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     offset_groups(a.ngroups, a.gstart);
     a.ntasks = s.ncells * a.ngroups;
     a.orig = s.orig; a.excl = s.excl;
-    const bool f32 = s.flags & MDRL_ACC_FP32, par = s.flags & MDRL_EVAL_PARITY,
-               ana = s.flags & MDRL_EVAL_ANALYTIC, coul = s.coul;
     const int str = coul ? 6 : 4;
     const size_t lds = ana ? 0 : (size_t)s.win.n * str * sizeof(float);
-    const int grid_n = s.sched.persistent ? std::min(a.ntasks, 2 * s.cu_count) : a.ntasks;
+    const int ntot = a.ntasks + a.bd.nblocks;
+    const int grid_n = s.sched.persistent ? std::min(ntot, 2 * s.cu_count) : ntot;
     const dim3 grid(grid_n);
     const Schedule &sc = s.sched;
     if (ana) {

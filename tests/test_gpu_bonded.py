@@ -31,11 +31,11 @@ def apoa1():
     return {k: t.get(k) for k in range(6)}
 
 
-def oracle_forces(s, apoa1, pos4=None):
+def oracle_forces(s, apoa1, pos4=None, mode=0):
     ncell = [int(math.floor(float(s.box[d]) / s.rc)) for d in range(3)]
     x = s.particle[:, :3] if pos4 is None else pos4[:, :3]
     rl = O.RL(x, s.box, ncell, s.rc, s.type, s.eps, s.sigma, apoa1_tab6(apoa1),
-              charge=s.charge, coulomb_k=s.coulomb_k, excl=s.excl)
+              charge=s.charge, coulomb_k=s.coulomb_k, excl=s.excl, mode=mode)
     fix, npairs = rl.forces_fix()
     bfix, te = O.bonded_fix(rl.pos4, s.box, s.term_atoms, s.term_params)
     tot = (fix.view(np.uint64) + bfix.view(np.uint64)).view(np.int64)
@@ -72,6 +72,41 @@ def test_chain_forces_every_schedule(apoa1):
         bad = np.nonzero(np.any(got != want, axis=1))[0]
         assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
     g.close()
+
+
+@pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])
+def test_chain_forces_without_exclusions(apoa1, mode):
+    """Bonded terms on a system without exclusion windows, in each pair mode (the bonded
+    blocks then run as a kernel of their own beside the force kernel): int64 forces and
+    pair counts equal the oracle's, and the total force is exactly zero."""
+    s = mdrl.SyntheticSystem(5, 1)
+    s.excl = None
+    flags = {"fast": 0, "parity": mdrl.EVAL_PARITY, "analytic": mdrl.EVAL_ANALYTIC}[mode]
+    omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
+    say("chains without exclusions: mode %s" % mode)
+    g = mdrl.System.from_synthetic(s, flags=flags)
+    got = g.forces_fixed()
+    npairs = g.stats().pair_evals
+    g.close()
+    want, wpairs, _, _ = oracle_forces(s, apoa1, mode=omode)
+    assert npairs == wpairs
+    bad = np.nonzero(np.any(got != want, axis=1))[0]
+    assert bad.size == 0, "atoms differ: %s" % bad[:10]
+    assert np.all(got.sum(axis=0) == 0)
+
+
+def test_chain_forces_fp32_atomics(apoa1):
+    """FP32-atomics accumulation with bonded terms (no exclusion windows): within 1e-5
+    (relative L2) of the int64 forces."""
+    s = mdrl.SyntheticSystem(5, 1)
+    s.excl = None
+    a = mdrl.System.from_synthetic(s)
+    fa = a.forces_fixed().astype(np.float64)
+    a.close()
+    b = mdrl.System.from_synthetic(s, flags=mdrl.ACC_FP32)
+    fb = b.forces_fixed().astype(np.float64)
+    b.close()
+    assert np.linalg.norm(fa - fb) / np.linalg.norm(fa) < 1e-5
 
 
 def test_chain_steps_bit_exact(apoa1):
