@@ -354,3 +354,82 @@ def test_ragged_cells_bit_exact(apoa1, mode):
         bad = np.nonzero(np.any(got != want, axis=1))[0]
         assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
     g.close()
+
+
+def test_cell_over_sort_capacity(apoa1):
+    """7,951 atoms held as one cell (a 43 A box with an explicit 1 x 1 x 1 grid, above the
+    2,048 atoms the in-cell sort holds): the gather keeps the scatter order and raises
+    MDRL_F_CELL_OVERFLOW, and the int64 forces, which do not depend on order, equal the
+    oracle's before and after two steps."""
+    s = mdrl.SyntheticSystem(4, 24)
+    say("one-cell system: n=%d box %.1f" % (s.n, s.box[0]))
+    g = mdrl.System(s.n, s.box, s.rc, s.eps, s.sigma, s.mass, dt=s.dt, coulomb_k=s.coulomb_k,
+                    ncell=(1, 1, 1))
+    g.set_types(s.type, s.charge)
+    g.set_particles(s.particle)
+    got = g.forces_fixed()
+    st = g.stats()
+    assert st.max_cell_count == s.n > 2048
+    assert st.flags & mdrl.F_CELL_OVERFLOW
+    tab = apoa1_tab6(apoa1)
+    rl = O.RL(s.particle[:, :3], s.box, (1, 1, 1), s.rc, s.type, s.eps, s.sigma, tab,
+              charge=s.charge, coulomb_k=s.coulomb_k)
+    want, npairs = rl.forces_fix()
+    assert st.pair_evals == npairs
+    assert np.array_equal(got, want)
+    g.step(2)
+    p = g.particles()
+    rl = O.RL(p[:, :3], s.box, (1, 1, 1), s.rc, s.type, s.eps, s.sigma, tab,
+              charge=s.charge, coulomb_k=s.coulomb_k)
+    want, npairs = rl.forces_fix()
+    assert g.stats().pair_evals == npairs
+    assert np.array_equal(g.forces_fixed(), want)
+    g.close()
+
+
+@pytest.mark.parametrize("mode", ["fast", "analytic"])
+def test_saturated_pair_force(apoa1, mode):
+    """Two atoms 0.2 A apart: F * dx * 2^32 exceeds 2^62, so the x components are clamped to
+    -+2^62 exactly as the oracle clamps them, y and z stay 0, and MDRL_F_SATURATED is
+    raised."""
+    s0 = mdrl.SyntheticSystem(1)
+    box = np.array([30.0, 30.0, 30.0], np.float32)
+    part = np.zeros((2, 11), np.float32)
+    part[0, :3] = [10.0, 10.0, 10.0]
+    part[1, :3] = [10.2, 10.0, 10.0]
+    types = np.zeros(2, np.int32)
+    flags = 0 if mode == "fast" else mdrl.EVAL_ANALYTIC
+    g = mdrl.System(2, box, s0.rc, s0.eps, s0.sigma, s0.mass, dt=s0.dt, flags=flags)
+    g.set_types(types)
+    g.set_particles(part)
+    got = g.forces_fixed()
+    st = g.stats()
+    g.close()
+    rl = O.RL(part[:, :3], box, (3, 3, 3), s0.rc, types, s0.eps, s0.sigma, apoa1_tab6(apoa1),
+              mode=0 if mode == "fast" else O.MODE_ANALYTIC)
+    want, npairs = rl.forces_fix()
+    assert npairs == 1 and st.pair_evals == 1
+    assert np.array_equal(got, want)
+    assert abs(int(got[0, 0])) == 1 << 62 and got[0, 0] == -got[1, 0]
+    assert np.all(got[:, 1:] == 0)
+    assert st.flags & mdrl.F_SATURATED
+
+
+def test_non_finite_states_reported(apoa1):
+    """A NaN position is refused at input, and a velocity that carries an atom to infinity
+    makes the step report MDRL_ENONFINITE (-6) instead of returning silently."""
+    s = mdrl.SyntheticSystem(1)
+    g = gpu_system(s)
+    bad = s.particle.copy()
+    bad[5, 1] = np.nan
+    with pytest.raises(mdrl.MdrlError) as e:
+        g.set_particles(bad)
+    assert e.value.code == -6
+    fast = s.particle.copy()
+    fast[7, 3] = 3e38
+    g.set_particles(fast)
+    with pytest.raises(mdrl.MdrlError) as e:
+        g.step(1)
+    assert e.value.code == -6
+    assert g.stats().flags & mdrl.F_NONFINITE
+    g.close()
