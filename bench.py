@@ -83,10 +83,39 @@ def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s):
                    % (cpu_model(), s.n, nref2 - 1, dt2, step_s, s.n, nref2, os.cpu_count())),
         "ms_per_step": step_s * 1e3,
     }
+    base["cmodel_agreement"] = cmodel_agreement(O)
     # force error vs the FP64 analytic oracle
     fd, _, _, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
                                 charge=s.charge if s.coulomb_k else None, ke=s.coulomb_k)
     return base, force_error(gpu_fix, fd), fd
+
+
+def cmodel_agreement(O, iterations=10, seed=2019):
+    """The reference C model's own computation (all pairs, pow() arithmetic, kick-drift) on
+    the GPU entry point against its CPU restatement: 500 argon atoms placed uniformly in the
+    C model's 24 A box from the seed, at rest, with the C model's defaults.  The C model
+    clears its force columns after each update, so the forces are compared through the
+    velocities after one iteration (v = F dt / m from rest); the whole state (positions,
+    velocities, per-atom energies) and the printed energy sums after `iterations`."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros((500, 11), np.float32)
+    p[:, :3] = rng.uniform(0.0, 24.0, (500, 3)).astype(np.float32)
+    out = {"input": "500 atoms uniform in the 24 A box, at rest (seed %d), C-model defaults" % seed,
+           "tolerance": 1e-5}
+    params = O.cmodel_default_params()
+    params.iterations = 1
+    want, _, _ = O.cmodel_run(p, params)
+    got, _ = mdrl.cmodel_run(p, iterations=1)
+    vw, vg = want[:, 3:6].astype(np.float64), got[:, 3:6].astype(np.float64)
+    rel = float(np.linalg.norm(vg - vw) / max(np.linalg.norm(vw), 1e-300))
+    out["force_rel_l2_1_iteration"] = rel
+    out["within_tolerance"] = rel <= 1e-5
+    params.iterations = iterations
+    want, we, _ = O.cmodel_run(p, params)
+    got, ge = mdrl.cmodel_run(p, iterations=iterations)
+    out["state_bit_identical_%d_iterations" % iterations] = bool(
+        np.array_equal(got.view(np.uint32), want.view(np.uint32)) and np.array_equal(ge.view(np.uint32), we.view(np.uint32)))
+    return out
 
 
 def force_error(gpu_fix, fd):
