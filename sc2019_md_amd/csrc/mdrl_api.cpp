@@ -863,7 +863,7 @@ extern "C" int md_rl_set_long_range(md_rl_ctx *c, md_rl_lr_fn fn, void *user)
 
 extern "C" int md_rl_commit(md_rl_ctx *c)
 {
-    if (!c) return MDRL_EINVAL;
+    if (!c) { set_error("md_rl_commit: bad argument"); return MDRL_EINVAL; }
     return c->dirty ? upload(c) : MDRL_OK;
 }
 
@@ -909,7 +909,7 @@ static void collect_times(md_rl_ctx *c)
 
 extern "C" int md_rl_force_eval(md_rl_ctx *c)
 {
-    if (!c) return MDRL_EINVAL;
+    if (!c) { set_error("md_rl_force_eval: bad argument"); return MDRL_EINVAL; }
     int rc = ensure_uploaded(c);
     if (rc) return rc;
     md_rl_ctx::StepEvents *ev = c->timing ? next_events(c) : nullptr;
@@ -944,10 +944,8 @@ static int enqueue_step(md_rl_ctx *c, hipStream_t st, bool events)
     return MDRL_OK;
 }
 
-// Two steps captured as one graph for the current buffer parity (the rebin swaps the
-// double buffers, so two steps return the pointers to where they started).
-// nsteps (even) steps captured as one graph for the current buffer parity; an even count
-// returns the double buffers to where they started.
+// nsteps (even) steps captured as one graph for the current buffer parity: each rebin swaps
+// the double buffers, so an even count returns the pointers to where they started.
 constexpr int kGraphLong = 16;
 static int capture_steps(md_rl_ctx *c, int nsteps, hipGraphExec_t *slot)
 {
@@ -969,7 +967,7 @@ static int capture_steps(md_rl_ctx *c, int nsteps, hipGraphExec_t *slot)
 
 extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
 {
-    if (!c || nsteps < 0) { set_error("md_rl_step: bad argument"); return MDRL_EINVAL; }
+    if (!c || nsteps < 0) { set_error("md_rl_step_async: bad argument"); return MDRL_EINVAL; }
     int rc = ensure_uploaded(c);
     if (rc) return rc;
     hipStream_t st = hip_stream ? (hipStream_t)hip_stream : c->stream;
@@ -995,7 +993,7 @@ extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
 
 extern "C" int md_rl_sync(md_rl_ctx *c)
 {
-    if (!c) return MDRL_EINVAL;
+    if (!c) { set_error("md_rl_sync: bad argument"); return MDRL_EINVAL; }
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     collect_times(c);
@@ -1173,7 +1171,7 @@ extern "C" int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *out)
 
 extern "C" int md_rl_set_timing(md_rl_ctx *c, int on)
 {
-    if (!c) return MDRL_EINVAL;
+    if (!c) { set_error("md_rl_set_timing: bad argument"); return MDRL_EINVAL; }
     c->timing = on != 0;
     c->tev_used = 0;
     return MDRL_OK;
@@ -1181,7 +1179,7 @@ extern "C" int md_rl_set_timing(md_rl_ctx *c, int on)
 
 extern "C" int md_rl_set_graph(md_rl_ctx *c, int on)
 {
-    if (!c) return MDRL_EINVAL;
+    if (!c) { set_error("md_rl_set_graph: bad argument"); return MDRL_EINVAL; }
     c->use_graph = on != 0;
     if (!c->use_graph) drop_graphs(c);
     return MDRL_OK;
@@ -1189,7 +1187,7 @@ extern "C" int md_rl_set_graph(md_rl_ctx *c, int on)
 
 extern "C" int md_rl_set_schedule(md_rl_ctx *c, const md_rl_schedule *sc)
 {
-    if (!c || !sc) return MDRL_EINVAL;
+    if (!c || !sc) { set_error("md_rl_set_schedule: bad argument"); return MDRL_EINVAL; }
     Schedule s{sc->atoms_per_wave, sc->tile_depth, sc->offset_groups, sc->persistent};
     if (!schedule_supported(c->s, s)) {
         set_error("schedule (apw %d, tile %d, groups %d) not available for this system's flags",
@@ -1204,7 +1202,7 @@ extern "C" int md_rl_set_schedule(md_rl_ctx *c, const md_rl_schedule *sc)
 
 extern "C" int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *sc)
 {
-    if (!c || !sc) return MDRL_EINVAL;
+    if (!c || !sc) { set_error("md_rl_get_schedule: bad argument"); return MDRL_EINVAL; }
     sc->atoms_per_wave = c->s.sched.apw;
     sc->tile_depth = c->s.sched.tile;
     sc->offset_groups = c->s.sched.groups;
@@ -1214,7 +1212,7 @@ extern "C" int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *sc)
 
 extern "C" int md_rl_autotune(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms)
 {
-    if (!c || reps < 1) return MDRL_EINVAL;
+    if (!c || reps < 1) { set_error("md_rl_autotune: bad argument"); return MDRL_EINVAL; }
     int rc = ensure_uploaded(c);
     if (rc) return rc;
     const bool was_timing = c->timing;

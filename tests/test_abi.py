@@ -77,6 +77,30 @@ def test_create_rejects_bad_parameters():
     assert e.value.code == -1
 
 
+def test_null_context_rejected_with_a_message():
+    """Every context entry point refuses a NULL context with MDRL_EINVAL and names itself
+    in the error message (no GPU needed)."""
+    L = mdrl.lib()
+    L.md_rl_last_error.restype = ctypes.c_char_p
+    calls = {
+        "md_rl_commit": lambda: L.md_rl_commit(None),
+        "md_rl_force_eval": lambda: L.md_rl_force_eval(None),
+        "md_rl_sync": lambda: L.md_rl_sync(None),
+        "md_rl_set_timing": lambda: L.md_rl_set_timing(None, 1),
+        "md_rl_set_graph": lambda: L.md_rl_set_graph(None, 1),
+        "md_rl_set_schedule": lambda: L.md_rl_set_schedule(None, None),
+        "md_rl_get_schedule": lambda: L.md_rl_get_schedule(None, None),
+        "md_rl_autotune": lambda: L.md_rl_autotune(None, 1, None, None),
+        "md_rl_get_stats": lambda: L.md_rl_get_stats(None, None),
+        "md_rl_energy": lambda: L.md_rl_energy(None, None, None, None),
+        "md_rl_step_async": lambda: L.md_rl_step_async(None, 1, None),
+    }
+    for name, call in calls.items():
+        assert call() == -1, name
+        assert name.encode() in L.md_rl_last_error(), name
+    assert L.md_rl_destroy(None) == 0
+
+
 def test_c_host_builds_against_the_header(tmp_path):
     """The plain C host (sc2019_md_amd/examples/host_c.c) compiles as strict C99 with
     -Wall -Wextra -pedantic -Werror against include/mdrl.h and links against libmdrl.so."""
