@@ -58,12 +58,13 @@ def test_chain_forces_bit_exact(apoa1, index):
 
 
 def test_chain_forces_every_schedule(apoa1):
-    """Exclusion windows under every force-kernel schedule (including the home-set
-    culling of 16 and 32 atoms per wave): int64 forces and pair counts equal the oracle's."""
+    """Exclusion windows and bonded blocks (tasks of the force kernel on this path) under
+    every force-kernel schedule, persistent or one workgroup per task: int64 forces and pair
+    counts equal the oracle's."""
     s = mdrl.SyntheticSystem(5, 0)
     want, wpairs, _, _ = oracle_forces(s, apoa1)
     g = mdrl.System.from_synthetic(s)
-    for sc in [(a, t, gr, 1) for a in (16, 32, 64) for t in (128, 256) for gr in (1, 4)]:
+    for sc in [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256) for gr in (1, 4) for p in (0, 1)]:
         say("chains: schedule", sc)
         g.set_schedule(*sc)
         g.force_eval()
