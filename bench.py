@@ -37,6 +37,7 @@ WORKLOADS = {
     1: "512-atom argon, 34 A box, rc 8.5 A, 4x4x4 cells, dt 2 fs",
     2: "92,224-atom LJ+Coulomb, 108.86x108.86x77.76 A, rc 12 A, 9x9x6 cells, dt 1 fs",
     3: "1,048,576-atom LJ+Coulomb, 219.0 A cube, rc 12 A, 18x18x18 cells, dt 1 fs",
+    4: "512 replicas, log-uniform 256..16,384 atoms each, 0.1 atoms/A^3, rc 10 A, LJ+Coulomb, dt 1 fs",
     5: "128 replicas of 64 chains x 48 united-atom beads, bonded + LJ + Coulomb, dt 2 fs",
 }
 
@@ -302,11 +303,12 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     return out
 
 
-def bench_bonded(args):
-    """The bonded workload: 128 replicas of 64 chains x 48 beads, bonded terms, exclusions,
-    range-limited LJ + Coulomb and velocity Verlet, one graph-captured batch."""
-    nrep = args.replicas if args.replicas else 128
-    systems = [mdrl.SyntheticSystem(5, k, seed=args.seed) for k in range(nrep)]
+def bench_batch(args, which):
+    """A batched workload as one graph-captured batch: the 512 uneven LJ + Coulomb replicas
+    (which = 4), or the bonded workload, 128 replicas of 64 chains x 48 beads with bonded
+    terms and exclusions (which = 5); then the same replicas as separate contexts."""
+    nrep = args.replicas if args.replicas else (128 if which == 5 else 512)
+    systems = [mdrl.SyntheticSystem(which, k, seed=args.seed) for k in range(nrep)]
     if args.dt is not None:
         for s in systems:
             s.dt = float(args.dt)
@@ -363,11 +365,13 @@ def bench_bonded(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded chains, seed %d)" % args.seed,
+        "data": "synthetic (seeded %s, seed %d)" % ("chains" if which == 5 else "lattice replicas", args.seed),
         "config": {
-            "workload": "%d replicas x 64 chains x 48 beads (%d beads), 0.033 beads/A^3, rc 10 A, "
-                        "bonds/angles/torsions, LJ + Coulomb excluded within 3 bonds, dt %g fs"
-                        % (nrep, natoms, s0.dt),
+            "workload": ("%d replicas x 64 chains x 48 beads (%d beads), 0.033 beads/A^3, rc 10 A, "
+                         "bonds/angles/torsions, LJ + Coulomb excluded within 3 bonds, dt %g fs"
+                         % (nrep, natoms, s0.dt)) if which == 5 else
+                        ("%d replicas (%d atoms), log-uniform 256..16,384 atoms each, 0.1 atoms/A^3, "
+                         "rc 10 A, LJ + Coulomb, dt %g fs" % (nrep, natoms, s0.dt)),
             "n_atoms": natoms,
             "bonded_terms": int(sum(s.nterms for s in systems)),
             "pair_evals_per_step": pairs,
@@ -408,7 +412,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replicas", type=int, default=None,
-                    help="replicas of the batched legs (default 512 for workload 2, 128 for 5; 0: skip)")
+                    help="replicas of the batched legs (default 512 for workloads 2 and 4, 128 for 5; 0: skip)")
     ap.add_argument("--replica-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=1, help="capture the timed steps in hipGraphs")
     ap.add_argument("--dt", type=float, default=None, help="time step in fs (default: the workload's)")
@@ -427,8 +431,8 @@ def main():
 
     if args.replicas is None:
         args.replicas = 128 if args.workload == 5 else 512
-    if args.workload == 5:
-        bench_bonded(args)
+    if args.workload in (4, 5):
+        bench_batch(args, args.workload)
         return
     s = mdrl.SyntheticSystem(args.workload, seed=args.seed)
     if args.dt is not None:
