@@ -10,11 +10,14 @@ step <= 1e-4 A^2 mol/kJ), then integrates velocity Verlet in FP64 with either th
 analytic forces (default) or the table forces of the force path (--tables), and prints
 potential, kinetic and total energy every 25 steps.
 
-Result for replica 40 (1,468 atoms): with FP64 analytic forces at dt = 1 fs the total
-energy holds near -7.7e4 kJ/mol for ~225 steps and then diverges (|E| > 1e40 by step
-325); at dt = 0.5 fs the FP64 and table trajectories agree and stay bounded.  The
-workload, not the force arithmetic, sets the stability limit: the sigma = 1 A species
-carries charges of up to ~1.3 e and its Coulomb-bound contacts are too stiff for 1 fs.
+Result for replica 40 (1,468 atoms) with the workload's masses (39.948 amu for every type):
+at dt = 1 fs the trajectory stays bounded over 400 steps (total energy -7.25e4 ->
+-8.26e4 kJ/mol while the strained relaxed lattice releases its energy).  The same replica
+with carbon-like masses (12-16 amu) also stays bounded over 600 steps, but the
+92,224-atom system with those masses diverges within 50 steps on the force path: among
+92,224 atoms the N(0, 0.4^2) charges reach ~1.8 e, and the Coulomb-bound contacts of the
+sigma = 1 A type become too stiff for velocity Verlet at 1 fs (omega*dt > 2).  The heavier
+masses lower omega by sqrt(39.948/12.011) = 1.8.
 """
 import argparse
 import math
