@@ -31,6 +31,7 @@ ACC_FP32, EVAL_ANALYTIC, EVAL_PARITY, KICK_DRIFT, NO_COULOMB = 1, 2, 4, 8, 16
 # sticky kernel flags (Stats.flags)
 F_SATURATED, F_NONFINITE, F_CELL_OVERFLOW = 1, 2, 4
 TERM_14, TERM_8, TERM_3, TERM_12, TERM_6, TERM_1 = range(6)
+NTERMS = 6
 
 _lib = None
 
@@ -125,6 +126,7 @@ def lib():
         L.md_rl_table_get.argtypes = [P, I, ctypes.POINTER(F), ctypes.POINTER(F),
                                       ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
         L.md_rl_table_free.argtypes = [P]
+        L.md_rl_table_from_arrays.argtypes = [ctypes.POINTER(P), I, I, ctypes.POINTER(F), ctypes.POINTER(F)]
         L.md_rl_table_free.restype = None
         L.md_rl_eval_pairs.argtypes = [P, P, P, F, F, F, F, F, F, F, F, F, ctypes.c_int64,
                                        ctypes.c_float, I]
@@ -211,6 +213,21 @@ class Table:
             _check(lib().md_rl_table_load_default(ctypes.byref(h)))
         else:
             _check(lib().md_rl_table_load(ctypes.byref(h), directory.encode(), exp_offset, nseg))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, c0, c1, exp_offset=121, nseg=14):
+        """A table from per-term coefficient arrays (c0[k], c1[k]: nseg*256 floats each, or
+        None where the term is absent), in MDRL_TERM_* order (md_rl_table_from_arrays)."""
+        F = ctypes.POINTER(ctypes.c_float)
+        p0, p1, keep = (F * NTERMS)(), (F * NTERMS)(), []
+        for k in range(NTERMS):
+            if c0[k] is not None and c1[k] is not None:
+                a0, a1 = _f32(c0[k], (nseg * 256,)), _f32(c1[k], (nseg * 256,))
+                keep += [a0, a1]
+                p0[k], p1[k] = _fp(a0), _fp(a1)
+        h = ctypes.c_void_p()
+        _check(lib().md_rl_table_from_arrays(ctypes.byref(h), exp_offset, nseg, p0, p1))
         return cls(h)
 
     def get(self, term):

@@ -107,3 +107,21 @@ def test_c_host_builds_against_the_header(tmp_path):
     import sc2019_md_amd.build as B
     out = B.build_host_example(out=str(tmp_path / "host_c"))
     assert os.path.exists(out) and os.access(out, os.X_OK)
+
+
+def test_table_from_arrays():
+    """md_rl_table_from_arrays keeps the caller's coefficients, leaves absent terms absent
+    and refuses a table without the r^-14 and r^-8 terms or with a bad geometry."""
+    t = mdrl.Table.load()
+    c0 = [t.get(k)[0] for k in range(mdrl.NTERMS)]
+    c1 = [t.get(k)[1] for k in range(mdrl.NTERMS)]
+    u = mdrl.Table.from_arrays(c0, c1)
+    for k in range(mdrl.NTERMS):
+        a0, a1 = u.get(k)
+        assert np.array_equal(a0, c0[k]) and np.array_equal(a1, c1[k])
+    part = mdrl.Table.from_arrays(c0[:2] + [None] * 4, c1[:2] + [None] * 4)
+    assert part.get(mdrl.TERM_3) == (None, None) and part.get(mdrl.TERM_14)[0] is not None
+    with pytest.raises(mdrl.MdrlError):
+        mdrl.Table.from_arrays([None] + c0[1:], [None] + c1[1:])
+    with pytest.raises(mdrl.MdrlError):
+        mdrl.Table.from_arrays(c0, c1, exp_offset=250, nseg=14)
