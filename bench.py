@@ -52,7 +52,7 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s):
+def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s, checks=None):
     """The C model restatement (oracle), one thread, on a bounded sample of the workload, and
     the GPU forces checked against the FP64 direct oracle.  The oracle is the checker and the
     CPU baseline here, never the measured GPU path."""
@@ -88,7 +88,16 @@ def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s):
     # force error vs the FP64 analytic oracle
     fd, _, _, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
                                 charge=s.charge if s.coulomb_k else None, ke=s.coulomb_k)
-    return base, force_error(gpu_fix, fd), fd
+    # further GPU force sets at other positions (the precision leg's relaxed system):
+    # relative L2 error of each against the FP64 direct evaluation there
+    extra = {}
+    if checks:
+        pos, sets = checks
+        fp, _, _, _ = O.fp64_direct(pos[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                    charge=s.charge if s.coulomb_k else None, ke=s.coulomb_k)
+        for name, f in sets.items():
+            extra[name] = float(np.linalg.norm(f.astype(np.float64) / 2 ** 32 * 0.1 - fp) / np.linalg.norm(fp))
+    return base, force_error(gpu_fix, fd), fd, extra
 
 
 def cmodel_agreement(O, iterations=10, seed=2019):
@@ -241,10 +250,10 @@ def batched_vs_plain(nrep, steps, seed, table):
 
 def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     """The precision configuration: int64 forces bit-identical over 10 repeat
-    evaluations, FP32 atomics vs int64 and both vs the FP64 direct oracle, FP64-reduced
-    total energy along NVE runs (one per time step in dt_list)."""
+    evaluations, FP32 atomics vs int64, FP64-reduced total energy along NVE runs (one per
+    time step in dt_list).  Returns the leg's record and, for the CPU leg's comparison with
+    the FP64 direct oracle, the relaxed positions and both force sets."""
     import hashlib
-    from oracle import oracle as O
 
     g = mdrl.System.from_synthetic(s)
     if relax:
@@ -262,17 +271,12 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     h.force_eval()
     f32b = h.forces_fixed()
     h.close()
-    fd, _, _, _ = O.fp64_direct(relaxed[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
-                                charge=s.charge, ke=s.coulomb_k)
-    rel = lambda a: float(np.linalg.norm(a.astype(np.float64) / 2 ** 32 * 0.1 - fd) / np.linalg.norm(fd))
     out = {
         "repeat_force_evals": 10,
         "int64_bit_identical": len(digests) == 1,
         "fp32_atomics_runs_bit_identical": bool(np.array_equal(f32a, f32b)),
         "fp32_vs_int64_rel_l2": float(np.linalg.norm((f32a - fix).astype(np.float64)) /
                                       np.linalg.norm(fix.astype(np.float64))),
-        "int64_vs_fp64_rel_l2": rel(fix),
-        "fp32_vs_fp64_rel_l2": rel(f32a),
         "nve": [],
     }
     for dt in dt_list:
@@ -300,7 +304,7 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
                            "rel_drift": (trace[-1][1] - e0) / abs(e0),
                            "max_rel_dev": max(abs(e - e0) for _, e in trace) / abs(e0),
                            "trace": trace, "wall_s": wall})
-    return out
+    return out, relaxed, {"int64": fix, "fp32": f32a}
 
 
 def bench_batch(args, which):
@@ -547,11 +551,16 @@ def main():
     if args.replicas and args.workload == 2:
         out["batched_vs_plain"] = batched_vs_plain(args.replicas, args.replica_steps, args.seed,
                                                    mdrl.Table.load())
+    checks = None
     if args.workload == 3:
-        out["precision"] = precision_leg(s, relax, args.relax_step, args.nve_steps,
-                                         sorted({s.dt, 0.5}))
+        out["precision"], relaxed_p, sets = precision_leg(s, relax, args.relax_step, args.nve_steps,
+                                                          sorted({s.dt, 0.5}))
+        checks = (relaxed_p, sets)
     if not args.no_cpu:
-        base, err, fd = cpu_baseline_leg(s, fix0, pairs, args.cpu_seconds)
+        base, err, fd, extra = cpu_baseline_leg(s, fix0, pairs, args.cpu_seconds, checks)
+        if "int64" in extra:
+            out["precision"]["int64_vs_fp64_rel_l2"] = extra["int64"]
+            out["precision"]["fp32_vs_fp64_rel_l2"] = extra["fp32"]
         out["cpu_baseline"] = base
         out["cpu_celllist_baseline"] = cpu_celllist_leg(s, pairs, args.cpu_seconds / 3)
         out["force_error"] = err
