@@ -255,6 +255,34 @@ __device__ __forceinline__ double bonded_term(const BondedArgs &a, int t, const 
     return e;
 }
 
+// One block of bonded terms (first term, end term, first atom, atom span), evaluated by
+// the threads t0, t0 + nthr, ... of a workgroup into the LDS accumulators acc (3 words per
+// atom of the span: fixed point, or FP32 in FP32-atomics mode) and then added into the
+// force buffer with one atomic per touched atom and component.  barrier() is the
+// workgroup barrier between the phases.
+template <typename Barrier>
+__device__ __forceinline__ void bonded_block(const BondedArgs &a, int4 blk, unsigned long long *acc, bool f32,
+                                             long long *fix, float *fbuf, int t0, int nthr, unsigned &flag,
+                                             Barrier barrier)
+{
+    float *accf = reinterpret_cast<float *>(acc);
+    for (int k = t0; k < 3 * kBondedBlockSpan; k += nthr) acc[k] = 0ull;
+    barrier();
+    const BondedSink sink{f32 ? nullptr : acc, f32 ? accf : nullptr, blk.z};
+    for (int t = blk.x + t0; t < blk.y; t += nthr) bonded_term<false>(a, t, sink, flag);
+    barrier();
+    for (int k = t0; k < 3 * blk.w; k += nthr) {
+        const int slot = a.slot_of[blk.z + k / 3];
+        if (f32) {
+            const float v = accf[k];
+            if (v != 0.0f) atomic_add_agent(&fbuf[4 * (size_t)slot + k % 3], v);
+        } else {
+            const unsigned long long v = acc[k];
+            if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(fix + 3 * (size_t)slot + k % 3), v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // k_rl_halfshell: one workgroup of 12 waves per task = (home cell, group of half-shell
 // offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
@@ -369,22 +397,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     // (3 x kBondedBlockSpan words: fixed point, or FP32 in FP32-atomics mode) and reach HBM
     // with one atomic per touched atom and component.
     auto bonded_task = [&](int b) {
-        const int4 blk = a.bd.blocks[b];   // first term, end term, first atom, atom span
-        for (int k = tid; k < 3 * kBondedBlockSpan; k += NTHR) s_iacc[k] = 0ull;
-        lds_barrier();
-        const BondedSink sink{F32 ? nullptr : s_iacc, F32 ? s_iacc_f : nullptr, blk.z};
-        for (int t = blk.x + tid; t < blk.y; t += NTHR) bonded_term<false>(a.bd, t, sink, flag);
-        lds_barrier();
-        for (int k = tid; k < 3 * blk.w; k += NTHR) {
-            const int slot = a.bd.slot_of[blk.z + k / 3];
-            if (F32) {
-                const float v = s_iacc_f[k];
-                if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)slot + k % 3], v);
-            } else {
-                const unsigned long long v = s_iacc[k];
-                if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot + k % 3), v);
-            }
-        }
+        bonded_block(a.bd, a.bd.blocks[b], s_iacc, F32, a.fix, a.f32, tid, NTHR, flag, [] { lds_barrier(); });
     };
     // bonded tasks are compiled into the exclusion variants only (the others keep their
     // registers); the launcher runs k_bonded beside the others
@@ -704,23 +717,9 @@ __global__ __launch_bounds__(256) void k_bonded_energy(BondedArgs a)
 __global__ __launch_bounds__(256) void k_bonded(BondedArgs a)
 {
     __shared__ unsigned long long s_acc[3 * kBondedBlockSpan];
-    const int4 blk = a.blocks[blockIdx.x];   // first term, end term, first atom, atom span
-    for (int k = threadIdx.x; k < 3 * kBondedBlockSpan; k += blockDim.x) s_acc[k] = 0ull;
-    __syncthreads();
-    const BondedSink sink{a.f32 ? nullptr : s_acc, a.f32 ? reinterpret_cast<float *>(s_acc) : nullptr, blk.z};
     unsigned flag = 0;
-    for (int t = blk.x + threadIdx.x; t < blk.y; t += blockDim.x) bonded_term<false>(a, t, sink, flag);
-    __syncthreads();
-    for (int k = threadIdx.x; k < 3 * blk.w; k += blockDim.x) {
-        const int slot = a.slot_of[blk.z + k / 3];
-        if (a.f32) {
-            const float v = reinterpret_cast<const float *>(s_acc)[k];
-            if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)slot + k % 3], v);
-        } else {
-            const unsigned long long v = s_acc[k];
-            if (v) atomic_add_agent(reinterpret_cast<unsigned long long *>(a.fix + 3 * (size_t)slot + k % 3), v);
-        }
-    }
+    bonded_block(a, a.blocks[blockIdx.x], s_acc, a.f32 != nullptr, a.fix, a.f32, (int)threadIdx.x, (int)blockDim.x,
+                 flag, [] { __syncthreads(); });
     if (flag) __hip_atomic_fetch_or(&a.counters[2], (unsigned long long)flag, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
 }
