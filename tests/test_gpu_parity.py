@@ -5,7 +5,6 @@ golden pair records; FP32-atomics forces within 1e-5 relative L2 of the int64 fo
 (their order of summation is not fixed); the C model bit-exact in its printed energies.
 """
 import math
-import os
 
 import numpy as np
 import pytest
