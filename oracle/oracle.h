@@ -24,7 +24,20 @@
  *   or_fp64_direct                   -> two-atom closed forms (LJ minimum at
  *                                       2^(1/6) sigma, Coulomb k q q / r^2),
  *                                       brute-force vs cell-list agreement
- *   or_motion_update                 -> closed forms: free flight, constant force
+ *   or_motion_update,                -> closed forms: free flight, constant force
+ *   or_minimize_update
+ *   or_fix                           -> exact values: 1 -> 2^32, ties to even,
+ *                                       +-1e30 -> +-2^62
+ *   or_table_fp64                    -> within 1e-5 of or_rl_forces_fix, within 1e-4
+ *                                       of or_fp64_direct (the tables' own error)
+ *   or_cmodel_force_sample           -> two-atom closed form of the C model's force
+ *   or_cell_of                       -> through pair counts == brute force over all
+ *                                       images (any consistent rule with cells >= rc)
+ *   or_atan2f                        -> FP64 arctan2 on 4,000 points (< 5e-7), axes
+ *   or_bonded_fix, or_bonded_fp64    -> central finite differences of the FP64 energy,
+ *                                       single-term closed forms, minimum image
+ *   exclusion windows                -> brute-force count of non-excluded pairs and an
+ *                                       independent FP64 sum that skips them
  */
 #ifndef MDRL_ORACLE_H
 #define MDRL_ORACLE_H
