@@ -54,8 +54,8 @@ def cpu_model():
 
 def cpu_baseline_leg(s, gpu_fix, pairs_per_step, budget_s, checks=None):
     """The C model restatement (oracle), one thread, on a bounded sample of the workload, and
-    the GPU forces checked against the FP64 direct oracle.  The oracle is the checker and the
-    CPU baseline here, never the measured GPU path."""
+    the GPU forces checked against the FP64 direct oracle.  The oracle supplies the reference
+    results and the CPU baseline here, never the measured GPU path."""
     from oracle import oracle as O
 
     p = O.cmodel_default_params()

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libmdrl, its kernels or its
  * host code) includes, links or calls this code.  Only tests/, the smoke check and
- * bench.py's CPU-baseline leg load it, and there only as the checker or as the
+ * bench.py's CPU-baseline leg load it, and there only as the reference results or as the
  * timed CPU baseline.
  *
  * Every function cites the reference file:line it restates.  Path prefixes:
@@ -178,7 +178,7 @@ void or_motion_update(int n, float *pos4, float *vel3, const int64_t *fix,
 void or_minimize_update(int n, float *pos4, const int64_t *fix, double gamma, double dmax,
                         const float box[3]);
 
-/* ---- bonded terms (BASELINE workload 5) ---------------------------------------- */
+/* ---- bonded terms (the bonded-chain workload) ------------------------------------ */
 /* One term per row of atoms[nterms][4] (input indices, -1 pads): a bond (i, j, -1, -1),
  * an angle (i, j, k, -1) with vertex j, or a torsion (i, j, k, l).  params[nterms][4],
  * forces in kJ/mol/nm from positions in A (the constants carry the factor 10):
