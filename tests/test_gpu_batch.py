@@ -45,6 +45,27 @@ def test_batch_forces_match_oracle(apoa1):
     b.close()
 
 
+def test_batch_with_an_anisotropic_system(apoa1):
+    """A 3 x 5 x 7-cell system between two cubic replicas of one batch: its forces equal the
+    oracle's, and six batched steps equal the same system stepped alone."""
+    from tests.test_gpu_parity import anisotropic_system
+    systems = [mdrl.SyntheticSystem(4, 0), anisotropic_system(), mdrl.SyntheticSystem(4, 3)]
+    say("batch with a 3 x 5 x 7-cell system: %s atoms" % [s.n for s in systems])
+    b = mdrl.Batch.from_synthetic(systems)
+    for k, s in enumerate(systems):
+        want, _ = rl_from_synthetic(s, apoa1).forces_fix()
+        assert np.array_equal(b.system_forces_fixed(k), want), "system %d" % k
+    b.step(6)
+    for k, s in enumerate(systems):
+        g = mdrl.System.from_synthetic(s)
+        g.step(6)
+        want = g.particles()
+        g.close()
+        got = b.system_particles(k)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), "system %d" % k
+    b.close()
+
+
 def test_batch_steps_match_single_systems():
     systems = replicas(IDX[:6])
     b = mdrl.Batch.from_synthetic(systems)

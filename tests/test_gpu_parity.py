@@ -196,6 +196,70 @@ def test_species_counts_not_powers_of_two(apoa1, nt, mode):
     assert bad.size == 0, "atoms differ: %s" % bad[:10]
 
 
+def anisotropic_system():
+    """The mixed-species parameters of a 5,710-atom replica on a 34 x 52 x 79 A box with
+    rc = 10 A: 3, 5 and 7 cells per axis, each longer than rc by a different amount.
+    The atoms sit on a 2.6 A lattice jittered by up to 0.2 A, with seeded types and
+    charges."""
+    s = mdrl.SyntheticSystem(4, 3)
+    s.box = np.array([34.0, 52.0, 79.0], np.float32)
+    rng = np.random.default_rng(357)
+    axes = [np.arange(int(s.box[d] // 2.6)) * 2.6 + 0.5 for d in range(3)]
+    grid = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    s.n = grid.shape[0]
+    s.particle = np.zeros((s.n, 11), np.float32)
+    s.particle[:, :3] = grid + rng.uniform(-0.2, 0.2, grid.shape)
+    s.particle[:, 3:6] = rng.normal(0.0, 0.04, (s.n, 3))   # A/fs
+    s.type = rng.integers(0, s.ntypes, s.n).astype(np.int32)
+    s.charge = rng.normal(0.0, 0.4, s.n).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])
+def test_anisotropic_box_forces(apoa1, mode):
+    """A different cell count, cell length and periodic wrap on each axis (3 x 5 x 7 cells,
+    sides not multiples of rc): int64 forces and the pair count equal the oracle's."""
+    s = anisotropic_system()
+    say("anisotropic box %s: n=%d mode=%s" % (s.box.tolist(), s.n, mode))
+    flags = {"fast": 0, "parity": mdrl.EVAL_PARITY, "analytic": mdrl.EVAL_ANALYTIC}[mode]
+    omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
+    g = gpu_system(s, flags)
+    got = g.forces_fixed()
+    npairs = g.stats().pair_evals
+    g.close()
+    rl = rl_from_synthetic(s, apoa1, omode)
+    assert list(rl.s.ncell) == [3, 5, 7]
+    want, wpairs = rl.forces_fix()
+    assert npairs == wpairs
+    bad = np.nonzero(np.any(got != want, axis=1))[0]
+    assert bad.size == 0, "atoms differ: %s" % bad[:10]
+
+
+def test_anisotropic_box_steps(apoa1):
+    """Ten steps on the 3 x 5 x 7-cell box (atoms cross cells and the periodic faces of
+    each axis): positions and forces bit-exact, step by step in the oracle."""
+    s = anisotropic_system()
+    say("anisotropic box steps: n=%d" % s.n)
+    g = gpu_system(s)
+    rl = rl_from_synthetic(s, apoa1)
+    hk = O.half_kick(s.mass, s.dt)
+    pos4 = rl.pos4.copy()
+    vel = s.particle[:, 3:6].copy()
+    fix, _ = rl.forces_fix()
+    for k in range(10):
+        pos4, vel, _ = O.motion_update(pos4, vel, fix, s.type, hk, s.mass, s.dt, s.box, k == 0)
+        rl.pos4[:] = pos4
+        fix, _ = rl.forces_fix()
+    wrapped = np.abs(pos4[:, :3] - s.particle[:, :3]) > s.box / 2
+    assert np.all(wrapped.sum(axis=0) > 0), "an axis without a periodic crossing"
+    g.step(10)
+    got = g.particles()
+    gfix = g.forces_fixed()
+    g.close()
+    assert np.array_equal(got[:, :3].view(np.uint32), pos4[:, :3].view(np.uint32))
+    assert np.array_equal(gfix, fix)
+
+
 # ------------------------------------------------------------------------ steps
 @pytest.mark.parametrize("which,index,nsteps", [(1, 0, 20), (4, 5, 10), (2, 0, 3)])
 def test_steps_bit_exact(apoa1, which, index, nsteps):
