@@ -48,7 +48,7 @@ def test_batch_forces_match_oracle(apoa1):
 def test_batch_with_an_anisotropic_system(apoa1):
     """A 3 x 5 x 7-cell system between two cubic replicas of one batch: its forces equal the
     oracle's, and six batched steps equal the same system stepped alone."""
-    from tests.test_gpu_parity import anisotropic_system
+    from tests.test_oracle_golden import anisotropic_system
     systems = [mdrl.SyntheticSystem(4, 0), anisotropic_system(), mdrl.SyntheticSystem(4, 3)]
     say("batch with a 3 x 5 x 7-cell system: %s atoms" % [s.n for s in systems])
     b = mdrl.Batch.from_synthetic(systems)

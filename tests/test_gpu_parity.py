@@ -15,8 +15,8 @@ from sc2019_md_amd import mdrl
 pytestmark = pytest.mark.gpu
 
 from tests.test_oracle_golden import (  # noqa: E402
-    apoa1_tab6, cmodel_inputs, hex2f, parse_pdb_float32, parse_reference_output, pdb_text,
-    rl_from_synthetic)
+    anisotropic_system, apoa1_tab6, cmodel_inputs, hex2f, parse_pdb_float32,
+    parse_reference_output, pdb_text, rl_from_synthetic)
 
 
 @pytest.fixture(scope="module")
@@ -194,25 +194,6 @@ def test_species_counts_not_powers_of_two(apoa1, nt, mode):
     assert npairs == wpairs
     bad = np.nonzero(np.any(got != want, axis=1))[0]
     assert bad.size == 0, "atoms differ: %s" % bad[:10]
-
-
-def anisotropic_system():
-    """The mixed-species parameters of a 5,710-atom replica on a 34 x 52 x 79 A box with
-    rc = 10 A: 3, 5 and 7 cells per axis, each longer than rc by a different amount.
-    The atoms sit on a 2.6 A lattice jittered by up to 0.2 A, with seeded types and
-    charges."""
-    s = mdrl.SyntheticSystem(4, 3)
-    s.box = np.array([34.0, 52.0, 79.0], np.float32)
-    rng = np.random.default_rng(357)
-    axes = [np.arange(int(s.box[d] // 2.6)) * 2.6 + 0.5 for d in range(3)]
-    grid = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
-    s.n = grid.shape[0]
-    s.particle = np.zeros((s.n, 11), np.float32)
-    s.particle[:, :3] = grid + rng.uniform(-0.2, 0.2, grid.shape)
-    s.particle[:, 3:6] = rng.normal(0.0, 0.04, (s.n, 3))   # A/fs
-    s.type = rng.integers(0, s.ntypes, s.n).astype(np.int32)
-    s.charge = rng.normal(0.0, 0.4, s.n).astype(np.float32)
-    return s
 
 
 @pytest.mark.parametrize("mode", ["fast", "parity", "analytic"])

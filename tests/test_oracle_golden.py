@@ -212,6 +212,25 @@ def rl_from_synthetic(s, apoa1, mode=0):
                 charge=s.charge if s.coulomb_k else None, coulomb_k=s.coulomb_k, mode=mode)
 
 
+def anisotropic_system():
+    """The mixed-species parameters of a 5,710-atom replica on a 34 x 52 x 79 A box with
+    rc = 10 A: 3, 5 and 7 cells per axis, each longer than rc by a different amount.
+    The atoms sit on a 2.6 A lattice jittered by up to 0.2 A, with seeded types and
+    charges."""
+    s = mdrl.SyntheticSystem(4, 3)
+    s.box = np.array([34.0, 52.0, 79.0], np.float32)
+    rng = np.random.default_rng(357)
+    axes = [np.arange(int(s.box[d] // 2.6)) * 2.6 + 0.5 for d in range(3)]
+    grid = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    s.n = grid.shape[0]
+    s.particle = np.zeros((s.n, 11), np.float32)
+    s.particle[:, :3] = grid + rng.uniform(-0.2, 0.2, grid.shape)
+    s.particle[:, 3:6] = rng.normal(0.0, 0.04, (s.n, 3))   # A/fs
+    s.type = rng.integers(0, s.ntypes, s.n).astype(np.int32)
+    s.charge = rng.normal(0.0, 0.4, s.n).astype(np.float32)
+    return s
+
+
 def test_half_shell_newton_third_law_exact(apoa1):
     s = mdrl.SyntheticSystem(1)
     rl = rl_from_synthetic(s, apoa1)
@@ -228,6 +247,22 @@ def test_half_shell_pairs_equal_brute_force(apoa1):
     _, _, _, nbrute = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
                                     use_cells=False)
     assert npairs == nbrute
+
+
+def test_anisotropic_box_pairs_equal_brute_force(apoa1):
+    """3 x 5 x 7 cells on sides that are not multiples of rc: the half-shell pairs are the
+    brute-force pairs over all images, the forces sum to zero exactly, and they agree with
+    FP64 direct evaluation to the table model's error."""
+    s = anisotropic_system()
+    rl = rl_from_synthetic(s, apoa1)
+    assert list(rl.s.ncell) == [3, 5, 7]
+    fix, npairs = rl.forces_fix()
+    assert np.all(fix.sum(axis=0) == 0)
+    fd, _, _, nbrute = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                     charge=s.charge, ke=s.coulomb_k, use_cells=False)
+    assert npairs == nbrute
+    f = fix.astype(np.float64) / 2 ** 32 * 0.1
+    assert np.linalg.norm(f - fd) / np.linalg.norm(fd) < 1e-4
 
 
 @pytest.mark.parametrize("which,index", [(1, 0), (4, 3), (4, 11)])
