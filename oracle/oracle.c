@@ -692,7 +692,7 @@ int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *
         double et = 0.0;
         int64_t np = 0;
         /* full shell: every atom gathers from the 27 neighbour cell images */
-        #pragma omp parallel for schedule(dynamic, 64) reduction(+:et, np)
+        #pragma omp parallel for schedule(dynamic, 64) reduction(+:et, np, vir[:9])
         for (int i = 0; i < n; ++i) {
             int c = cid[i];
             int cx = c / (nc[1] * nc[2]), cy = (c / nc[2]) % nc[1], cz = c % nc[2];
@@ -724,6 +724,9 @@ int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *
                                  coulomb, &F, &E);
                     fi[0] += F * dx; fi[1] += F * dy; fi[2] += F * dz;
                     et += 0.5 * E;
+                    const double d3[3] = {dx, dy, dz};
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) vir[3 * a + b] += 0.5 * F * d3[a] * d3[b];
                     ++np;
                 }
             }

@@ -24,6 +24,8 @@
  *   or_fp64_direct                   -> two-atom closed forms (LJ minimum at
  *                                       2^(1/6) sigma, Coulomb k q q / r^2),
  *                                       brute-force vs cell-list agreement
+ *                                       (energy and virial); virial trace ==
+ *                                       -dU/dl under scaling by l
  *   or_motion_update,                -> closed forms: free flight, constant force
  *   or_minimize_update
  *   or_fix                           -> exact values: 1 -> 2^32, ties to even,
@@ -151,7 +153,8 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out);
 /* FP64 analytic oracle: LJ (Lorentz-Berthelot params given as eps/sig per type pair)
  * plus cutoff Coulomb, all periodic images within rc, forces in kJ/mol/A.
  * qs: plain charges (e).  Brute force over all pairs and the 27 nearest images, or the
- * cell list when use_cells is set.  Optional energy (kJ/mol) and virial (9). */
+ * cell list when use_cells is set.  Optional energy (kJ/mol) and pair virial
+ * W[3a+b] = sum over pairs of F d_a d_b (F d = force on i, d = r_i - r_j; 9 values). */
 int64_t or_fp64_direct(int n, const double *xyz, const int *type, const double *q,
                        int ntypes, const double *eps_ij, const double *sig_ij,
                        double ke, int coulomb, const double box[3], double rc,

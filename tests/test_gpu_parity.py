@@ -311,6 +311,25 @@ def test_energy_vs_fp64(apoa1):
     assert ke == pytest.approx(ke_want, rel=1e-12)
 
 
+@pytest.mark.parametrize("which,index", [(1, 0), (4, 3)])
+def test_virial_vs_fp64(which, index):
+    """The pair virial sum_ij F_ij (x) r_ij from the force tables, FP64-reduced on the GPU,
+    against the FP64 analytic oracle's (argon; a mixed-species replica with Coulomb). Its
+    trace is sum_ij F_ij . r_ij; the table model carries up to 1.1e-4 relative error."""
+    s = mdrl.SyntheticSystem(which, index)
+    g = gpu_system(s)
+    pe, _, vir = g.energy()
+    g.close()
+    q = s.charge if s.coulomb_k else None
+    _, e64, vir64, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                     charge=q, ke=s.coulomb_k)
+    say("virial: workload %d replica %d trace %.6e (FP64 %.6e)" % (which, index, np.trace(vir),
+                                                                 np.trace(vir64)))
+    assert pe == pytest.approx(e64, rel=2e-4)
+    assert np.linalg.norm(vir - vir64) / np.linalg.norm(vir64) < 2e-4
+    assert np.trace(vir) == pytest.approx(np.trace(vir64), rel=2e-4)
+
+
 def test_nve_energy_drift():
     """Argon, 2,000 steps of 2 fs.  The reference potential is truncated without a shift
     (the *_no_smooth tables), so the potential jumps by U(rc) whenever a pair crosses the

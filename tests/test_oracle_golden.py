@@ -265,6 +265,30 @@ def test_anisotropic_box_pairs_equal_brute_force(apoa1):
     assert np.linalg.norm(f - fd) / np.linalg.norm(fd) < 1e-4
 
 
+@pytest.mark.parametrize("which,index", [(1, 0), (4, 3)])
+def test_fp64_virial(which, index):
+    """The FP64 oracle's pair virial W = sum_ij F_ij (x) r_ij: the cell-list path equals the
+    brute-force path over all images, and (argon) its trace equals -dU/dl for positions and
+    box scaled by l, a central difference of the oracle's own energy."""
+    s = mdrl.SyntheticSystem(which, index)
+    q = s.charge if s.coulomb_k else None
+    args = (s.type, s.eps, s.sigma)
+    _, e, w, _ = O.fp64_direct(s.particle[:, :3], *args, s.box, s.rc, charge=q, ke=s.coulomb_k)
+    _, eb, wb, _ = O.fp64_direct(s.particle[:, :3], *args, s.box, s.rc, charge=q,
+                                 ke=s.coulomb_k, use_cells=False)
+    assert np.linalg.norm(w) > 0
+    assert eb == pytest.approx(e, rel=1e-12)
+    assert np.abs(w - wb).max() <= 1e-12 * np.abs(wb).max()
+    assert np.abs(w - w.T).max() <= 1e-12 * np.abs(w).max()
+    if which == 1:
+        # no pair lies within rc * 1e-7 of the cutoff, so none crosses it under the scaling
+        h = 1e-7
+        x = s.particle[:, :3].astype(np.float64)
+        up = O.fp64_direct(x * (1 + h), *args, s.box.astype(np.float64) * (1 + h), s.rc)[1]
+        dn = O.fp64_direct(x * (1 - h), *args, s.box.astype(np.float64) * (1 - h), s.rc)[1]
+        assert np.trace(w) == pytest.approx(-(up - dn) / (2 * h), rel=1e-6)
+
+
 @pytest.mark.parametrize("which,index", [(1, 0), (4, 3), (4, 11)])
 def test_half_shell_vs_fp64(apoa1, which, index):
     """Kernel arithmetic vs FP64 of the same table model (error (i)), and the table model vs
