@@ -146,12 +146,15 @@ def test_batch_energies_deterministic():
     e2 = [b.system_energy(k) for k in range(4)]
     for (p1, k1, v1), (p2, k2, v2) in zip(e1, e2):
         assert p1 == p2 and k1 == k2 and np.array_equal(v1, v2)
-    pe, ke, _ = b.energy()
+    pe, ke, vir = b.energy()
     assert pe == pytest.approx(sum(e[0] for e in e1), rel=1e-12)
+    vsum = sum(e[2] for e in e1)
+    assert np.abs(vir - vsum).max() <= 1e-12 * np.abs(vsum).max()
     for k, s in enumerate(systems):
-        _, e64, _, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
-                                     charge=s.charge, ke=s.coulomb_k)
+        _, e64, v64, _ = O.fp64_direct(s.particle[:, :3], s.type, s.eps, s.sigma, s.box, s.rc,
+                                       charge=s.charge, ke=s.coulomb_k)
         assert e1[k][0] == pytest.approx(e64, rel=2e-4, abs=1e-3 * abs(e64) + 1.0)
+        assert np.linalg.norm(e1[k][2] - v64) <= 1e-3 * np.linalg.norm(v64) + 1.0, "system %d" % k
     b.close()
 
 
