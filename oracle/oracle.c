@@ -278,7 +278,7 @@ int64_t or_pair_stream_text(const double *raw_xyz, int n, float rc, const int nc
     for (int i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d)
             if (raw_xyz[3 * i + d] < mn[d]) mn[d] = raw_xyz[3 * i + d];
-    float *pos = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+    float *pos = (float *)calloc(3 * (size_t)n, sizeof(float));
     int *cxyz = (int *)malloc(sizeof(int) * 3 * (size_t)n);
     for (int i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) pos[3 * i + d] = (float)(raw_xyz[3 * i + d] - mn[d]);
@@ -405,7 +405,7 @@ static void build_cells(int n, const float *pos4, const float box[3], const int 
         cl->count[cid[i]]++;
     }
     for (int c = 0; c < cl->ncells; ++c) cl->start[c + 1] = cl->start[c] + cl->count[c];
-    int *fill = (int *)calloc((size_t)cl->ncells, sizeof(int));
+    int *fill = (int *)calloc((size_t)(cl->ncells > 0 ? cl->ncells : 1), sizeof(int));
     for (int i = 0; i < n; ++i) cl->list[cl->start[cid[i]] + fill[cid[i]]++] = i;
     free(fill);
     free(cid);
