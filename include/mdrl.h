@@ -200,6 +200,12 @@ int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t *atoms, co
  * energies and pair counts.  Windows must be symmetric with bit 16 clear; NULL removes
  * them.  Needs the default int64 table path. */
 int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask);
+/* Host-only check of exclusion windows as md_rl_set_exclusions applies it (no counterpart
+ * in the reference): every set bit d of mask[i] names an atom i + d - 16 in [0, n) whose
+ * own window has bit 32 - d set; bit 16 (the atom itself) and bit 0 (atom i - 16, whose
+ * reciprocal bit 32 lies outside the window) are rejected.  Returns MDRL_OK or
+ * MDRL_EINVAL with the offending atom in md_rl_last_error(). */
+int md_rl_check_exclusions(const uint32_t *mask, int n);
 /* Summation hook for forces computed outside this library (a long-range solver, an
  * applied field, restraints).  The reference sums short-range, long-range and bonded
  * forces per particle before Motion_Update (Summation_Logic/Summation_Logic.v:3-6,

@@ -357,6 +357,15 @@ int64_t or_pair_stream_text(const double *raw_xyz, int n, float rc, const int nc
 /* half-shell table forces, int64 fixed point                                 */
 /* ------------------------------------------------------------------------- */
 
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int or_cell_of(float x, float inv_edge, int nc)
 {
     float t = x * inv_edge;

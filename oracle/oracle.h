@@ -139,6 +139,11 @@ int or_cell_of(float x, float inv_edge, int nc);
 /* Per-pair fixed-point conversion: rint(f * 2^32) saturated at +-2^62. */
 int64_t or_fix(float f);
 
+/* OpenMP threads of the calling thread's later oracle calls (n >= 1); callers that run many
+ * small systems side by side give each of their own threads one.  Integer sums make every
+ * result independent of the thread count. */
+void or_set_threads(int n);
+
 /* Half-shell forces: cells of edge L/ncell, home cell + the 13 offsets of
  * D/Particle_Pair_Gen_HalfShell_64_Cells.v:213-226 ({223,231,232,233,311,312,313,
  * 321,322,323,331,332,333} around 222), explicit image shifts, filter 0 < r2 <= rc2

@@ -64,6 +64,8 @@ def lib():
                                           ctypes.c_char_p, ctypes.c_int64]
         L.or_pair_stream_text.restype = ctypes.c_int64
         L.or_cell_of.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_set_threads.restype = None
         L.or_fix.argtypes = [ctypes.c_float]
         L.or_fix.restype = ctypes.c_int64
         L.or_rl_forces_fix.argtypes = [ctypes.POINTER(RLSystem), I64]
@@ -161,6 +163,28 @@ def cell_of(x, inv_edge, nc):
 
 def fix(f):
     return lib().or_fix(float(np.float32(f)))
+
+
+def set_threads(n):
+    """OpenMP threads of this (Python) thread's later oracle calls."""
+    lib().or_set_threads(int(n))
+
+
+def map_systems(fn, items, workers=None, cost=None):
+    """fn over items on a pool of threads, each running the oracle on one OpenMP thread
+    (many small systems side by side instead of one at a time over all cores), the most
+    costly items (cost(item), default its atom count n) submitted first; results in input
+    order."""
+    from concurrent.futures import ThreadPoolExecutor
+    workers = workers or min(16, os.cpu_count() or 1)
+    cost = cost or (lambda it: getattr(it, "n", 0))
+    order = sorted(range(len(items)), key=lambda k: -cost(items[k]))
+    out = [None] * len(items)
+    with ThreadPoolExecutor(max_workers=workers, initializer=set_threads, initargs=(1,)) as ex:
+        futs = {k: ex.submit(fn, items[k]) for k in order}
+        for k, f in futs.items():
+            out[k] = f.result()
+    return out
 
 
 # ---------------------------------------------------------------------------- host-side setup

@@ -804,6 +804,32 @@ extern "C" int md_rl_set_bonded(md_rl_ctx *c, int sys, int nterms, const int32_t
     return (c->s.nsys == 1 && c->have_particles) ? upload(c) : MDRL_OK;
 }
 
+extern "C" int md_rl_check_exclusions(const uint32_t *mask, int n)
+{
+    if (!mask || n < 0) { set_error("md_rl_check_exclusions: bad argument"); return MDRL_EINVAL; }
+    for (int i = 0; i < n; ++i) {
+        const uint32_t m = mask[i];
+        if (m & (1u << 16)) { set_error("md_rl_check_exclusions: atom %d excludes itself", i); return MDRL_EINVAL; }
+        // bit 0 names atom i - 16, whose bit for atom i would be bit 32: outside the
+        // window, so such a mask can never exclude back
+        if (m & 1u) {
+            set_error("md_rl_check_exclusions: window of atom %d names atom %d (bit 0), which "
+                      "the 32-bit window cannot exclude back", i, i - 16);
+            return MDRL_EINVAL;
+        }
+        for (int d = 1; d < 32; ++d) {   // the reciprocal bit 32 - d lies in 1..31
+            if (!((m >> d) & 1u)) continue;
+            const int j = i + d - 16;
+            if (j < 0 || j >= n || !((mask[j] >> (32 - d)) & 1u)) {
+                set_error("md_rl_check_exclusions: window of atom %d names atom %d, out of range or "
+                          "not excluding it back", i, j);
+                return MDRL_EINVAL;
+            }
+        }
+    }
+    return MDRL_OK;
+}
+
 extern "C" int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask)
 {
     if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_set_exclusions: bad argument"); return MDRL_EINVAL; }
@@ -814,19 +840,9 @@ extern "C" int md_rl_set_exclusions(md_rl_ctx *c, int sys, const uint32_t *mask)
     const SysGeom &G = c->s.h_geom[sys];
     std::vector<uint32_t> m;
     if (mask) {
+        int rc = md_rl_check_exclusions(mask, G.n);
+        if (rc) return rc;
         m.assign(mask, mask + G.n);
-        for (int i = 0; i < G.n; ++i) {
-            if (m[i] & (1u << 16)) { set_error("md_rl_set_exclusions: atom %d excludes itself", i); return MDRL_EINVAL; }
-            for (int d = 0; d < 32; ++d) {
-                if (!((m[i] >> d) & 1u)) continue;
-                const int j = i + d - 16;
-                if (j < 0 || j >= G.n || !((m[j] >> (32 - d)) & 1u)) {
-                    set_error("md_rl_set_exclusions: window of atom %d names atom %d, out of range or "
-                              "not excluding it back", i, j);
-                    return MDRL_EINVAL;
-                }
-            }
-        }
     }
     int rc = restage(c);
     if (rc) return rc;

@@ -14,6 +14,7 @@ struct md_rl_table {
     int nent = 0;                       // nseg * 256
     bool has[MDRL_NTERMS] = {};
     std::vector<float> c0[MDRL_NTERMS], c1[MDRL_NTERMS];
+    uint64_t id = 0;                    // unique per table (tables are immutable)
 };
 
 namespace mdrl {

@@ -9,6 +9,7 @@
 //         M/LJ_no_smooth_poly_interpolation_function.m:248-258)
 // .mif and .hex go through the strict memory-image reader of mdrl_cells.cpp; a table must
 // assign every entry, and a damaged file is an error, never a shorter or absent table.
+#include <atomic>
 #include <dlfcn.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -109,7 +110,9 @@ extern "C" int md_rl_table_from_arrays(md_rl_table **out, int exp_offset, int ns
         set_error("md_rl_table_from_arrays: the r^-14 and r^-8 terms are required");
         return MDRL_EINVAL;
     }
+    static std::atomic<uint64_t> next_id{1};
     md_rl_table *t = new md_rl_table;
+    t->id = next_id.fetch_add(1);
     t->exp_offset = exp_offset;
     t->nseg = nseg;
     t->nent = nseg * 256;

@@ -155,6 +155,7 @@ def lib():
         L.md_rl_set_system.argtypes = [P, I, F, ctypes.POINTER(ctypes.c_int32), F]
         L.md_rl_set_bonded.argtypes = [P, I, I, ctypes.POINTER(ctypes.c_int32), F]
         L.md_rl_set_exclusions.argtypes = [P, I, ctypes.POINTER(ctypes.c_uint32)]
+        L.md_rl_check_exclusions.argtypes = [ctypes.POINTER(ctypes.c_uint32), I]
         L.md_rl_commit.argtypes = [P]
         L.md_rl_get_system.argtypes = [P, I, F]
         L.md_rl_get_system_forces_fixed.argtypes = [P, I, ctypes.POINTER(ctypes.c_int64)]
@@ -553,6 +554,12 @@ class Batch(System):
         vir = (ctypes.c_double * 9)()
         _check(lib().md_rl_system_energy(self.handle, sys, ctypes.byref(pe), ctypes.byref(ke), vir))
         return pe.value, ke.value, np.array(vir[:]).reshape(3, 3)
+
+
+def check_exclusions(mask):
+    """Host-only check of exclusion windows (md_rl_check_exclusions); raises MdrlError."""
+    m = np.ascontiguousarray(mask, np.uint32)
+    _check(lib().md_rl_check_exclusions(m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), m.shape[0]))
 
 
 # ------------------------------------------------------------ cell-memory initialisation files

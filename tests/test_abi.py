@@ -125,3 +125,40 @@ def test_table_from_arrays():
         mdrl.Table.from_arrays([None] + c0[1:], [None] + c1[1:])
     with pytest.raises(mdrl.MdrlError):
         mdrl.Table.from_arrays(c0, c1, exp_offset=250, nseg=14)
+
+
+def chain_windows(n, reach=3):
+    """Symmetric exclusion windows of one linear chain: each atom excludes the atoms within
+    `reach` bonds on either side."""
+    m = np.zeros(n, np.uint32)
+    for i in range(n):
+        for j in range(max(0, i - reach), min(n, i + reach + 1)):
+            if j != i:
+                m[i] |= np.uint32(1) << np.uint32(j - i + 16)
+    return m
+
+
+def test_exclusion_windows_checked_on_the_host():
+    m = chain_windows(40)
+    mdrl.check_exclusions(m)                        # symmetric windows pass
+    bad = m.copy()
+    bad[5] |= np.uint32(1 << 16)                    # the atom itself
+    with pytest.raises(mdrl.MdrlError, match="excludes itself"):
+        mdrl.check_exclusions(bad)
+    bad = m.copy()
+    bad[30] |= np.uint32(1)                         # bit 0: atom 14, reciprocal bit 32
+    with pytest.raises(mdrl.MdrlError, match="bit 0"):
+        mdrl.check_exclusions(bad)
+    bad = m.copy()
+    bad[3] |= np.uint32(1 << 31)                    # atom 18, which does not exclude back
+    with pytest.raises(mdrl.MdrlError, match="not excluding it back"):
+        mdrl.check_exclusions(bad)
+    bad = m.copy()
+    bad[38] |= np.uint32(1 << 20)                   # atom 42, past the end
+    with pytest.raises(mdrl.MdrlError, match="out of range"):
+        mdrl.check_exclusions(bad)
+    # the same pair excluded from both ends with the largest offset the window allows
+    ok = np.zeros(40, np.uint32)
+    ok[2] |= np.uint32(1 << 31)                     # atom 17 = 2 + 31 - 16
+    ok[17] |= np.uint32(1 << 1)                     # atom 2 = 17 + 1 - 16
+    mdrl.check_exclusions(ok)
