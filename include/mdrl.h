@@ -129,6 +129,20 @@ int md_rl_eval_pairs_ex(const md_rl_table *t, const float *ref_xyz, const float 
                         const float *p_a, const float *p_b, const float *p_qq,
                         float *force_s, float *force_xyz, int64_t npairs, float rc2,
                         int flags);
+/* md_rl_eval_pairs_ex on device buffers, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  All pointers except t are
+ * device pointers (p_a, p_b, p_qq, force_s may be NULL as above); the work is enqueued on
+ * hip_stream (NULL: the null stream) and the call returns without waiting for it.  The
+ * interleaved table lives in a fixed scratch array of 384 KiB per GPU (tables of at most
+ * 32 segments; larger ones are rejected with MDRL_EINVAL), staged from t's host arrays
+ * when t differs from the table staged last on this GPU; that staging waits for its
+ * copies (so a call that stages cannot be captured into a graph: make one call with the
+ * table before capturing).  Calls that overlap on one GPU share its scratch and race. */
+int md_rl_eval_pairs_ex_noalloc(const md_rl_table *t, const float *ref_xyz,
+                                const float *nbr_xyz, const float *p_a, const float *p_b,
+                                const float *p_qq, float *force_s, float *force_xyz,
+                                int64_t npairs, float rc2, int flags, void *hip_stream);
 
 /* ---- system path ------------------------------------------------------------------ */
 /* Half-shell cell list (D/Particle_Pair_Gen_HalfShell_64_Cells.v), r2 filter

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmdrl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mdrl_system.hip", "mdrl_kernels.hip", "mdrl_api.cpp", "mdrl_io.cpp", "mdrl_cells.cpp", "mdrl_gen.cpp"]
+SOURCES = ["mdrl_system.hip", "mdrl_kernels.hip", "mdrl_noalloc.hip", "mdrl_api.cpp", "mdrl_io.cpp",
+           "mdrl_cells.cpp", "mdrl_gen.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950",

@@ -132,6 +132,8 @@ def lib():
                                        ctypes.c_float, I]
         L.md_rl_eval_pairs_ex.argtypes = [P, F, F, F, F, F, F, F, ctypes.c_int64,
                                           ctypes.c_float, I]
+        L.md_rl_eval_pairs_ex_noalloc.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int64,
+                                                  ctypes.c_float, I, P]
         L.md_rl_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(Params)]
         for name in ("md_rl_set_particles", "md_rl_get_particles"):
             getattr(L, name).argtypes = [P, F, I]
@@ -291,6 +293,14 @@ def eval_pairs_ex(table, ref, nbr, rc2, flags=0, p_a=None, p_b=None, p_qq=None):
                                      *[None if a is None else _fp(a) for a in arrs],
                                      _fp(fs), _fp(fx), n, rc2, flags))
     return fs, fx
+
+
+def eval_pairs_ex_noalloc(table, ref_ptr, nbr_ptr, npairs, rc2, fxyz_ptr, fs_ptr=None, flags=0,
+                          p_a_ptr=None, p_b_ptr=None, p_qq_ptr=None, stream=None):
+    """md_rl_eval_pairs_ex_noalloc on device pointers (integers, e.g. torch data_ptr());
+    enqueued on stream (a hipStream_t as an integer; None: the null stream)."""
+    _check(lib().md_rl_eval_pairs_ex_noalloc(table.handle, ref_ptr, nbr_ptr, p_a_ptr, p_b_ptr,
+                                             p_qq_ptr, fs_ptr, fxyz_ptr, npairs, rc2, flags, stream))
 
 
 # ---------------------------------------------------------------------------------- systems
