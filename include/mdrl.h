@@ -297,6 +297,12 @@ int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke, double vi
 /* Capture md_rl_step's steps in hipGraphs (16 steps per graph launch, then pairs of steps); off
  * by default. */
 int md_rl_set_graph(md_rl_ctx *c, int on);
+/* Captures, without running them, the graphs that md_rl_step / md_rl_step_async of nsteps
+ * steps would replay from the current state, so that those calls only replay graphs (no
+ * capture or instantiation inside them).  Needs md_rl_set_graph(c, 1) and a context that
+ * has taken its first step since its particles were set (that step runs eagerly);
+ * otherwise it captures nothing and returns MDRL_OK.  No reference counterpart. */
+int md_rl_graph_prepare(md_rl_ctx *c, int nsteps);
 
 /* Force-kernel schedule.  atoms_per_wave: home atoms per 64-lane wavefront (16, 32 or
  * 64; below 64, 64/apw lanes share a home atom and split its neighbours); tile_depth:

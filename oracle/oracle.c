@@ -476,6 +476,21 @@ static inline int or_excluded(const uint32_t *m, int i, int j)
     return (unsigned)d < 32u && ((m[i] >> d) & 1u);
 }
 
+/* The analytic mode's cutoff decided exactly: D = (x_i - t) - e per axis in FP64, with
+ * t = fl(x_j + S) the staged image and e = (x_j + S) - t its exact error, so D is the exact
+ * separation x_i - (x_j + S) (both differences exact in FP64 for positions in [0, L) and S
+ * in {-L, 0, L}); r^2 = (Dx^2 + Dy^2) + Dz^2 against rc^2 in FP64.  This is the membership of
+ * the FP64 direct evaluation, so the pair set carries no FP32 rounding. */
+static inline int or_within_cutoff_exact(const float *pi, const float t[3], const float e[3],
+                                         double rc2e)
+{
+    double D[3];
+    for (int d = 0; d < 3; ++d) D[d] = ((double)pi[d] - (double)t[d]) - (double)e[d];
+    double e2 = D[0] * D[0] + D[1] * D[1];
+    e2 = e2 + D[2] * D[2];
+    return e2 <= rc2e;
+}
+
 int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
 {
     const int n = s->n;
@@ -483,6 +498,8 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
     build_cells(n, s->pos4, s->box, s->ncell, &cl);
     memset(fix_out, 0, sizeof(int64_t) * 3 * (size_t)n);
     const int par = s->mode & OR_MODE_PARITY;
+    const int ana = s->mode & OR_MODE_ANALYTIC;
+    const double rc2e = (double)s->rc * (double)s->rc;
     int64_t npairs = 0;
     const int *nc = s->ncell;
     /* Home cells run in parallel, each thread into its own int64 accumulators; integer
@@ -520,7 +537,20 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
                     int j = cl.list[cl.start[c] + b];
                     const float *pj = s->pos4 + 4 * (size_t)j;
                     float dx, dy, dz, r2;
-                    if (par) {
+                    if (ana) {   /* analytic takes precedence over parity, as on the GPU */
+                        /* t = fl(x_j + S) as staged, e its exact error, dx = fl((x_i - t) - e) */
+                        float t[3], e[3], d[3];
+                        for (int k = 0; k < 3; ++k) {
+                            t[k] = pj[k] + S[k];
+                            const float z = t[k] - S[k];
+                            e[k] = pj[k] - z;
+                            d[k] = pi[k] - t[k];
+                            d[k] = d[k] - e[k];
+                        }
+                        dx = d[0]; dy = d[1]; dz = d[2];
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        if (!(r2 > 0.0f && or_within_cutoff_exact(pi, t, e, rc2e))) continue;
+                    } else if (par) {
                         dx = pi[0] - pj[0]; dx = dx - S[0];
                         dy = pi[1] - pj[1]; dy = dy - S[1];
                         dz = pi[2] - pj[2]; dz = dz - S[2];
@@ -531,7 +561,7 @@ int64_t or_rl_forces_fix(const or_rl_system *s, int64_t *fix_out)
                         dx = pi[0] - xj; dy = pi[1] - yj; dz = pi[2] - zj;
                         r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     }
-                    if (!(r2 > 0.0f && r2 <= s->rc2)) continue;
+                    if (!ana && !(r2 > 0.0f && r2 <= s->rc2)) continue;
                     if (s->excl && or_excluded(s->excl, i, j)) continue;
                     ++npairs;
                     float qq = pi[3] * pj[3];

@@ -112,7 +112,9 @@ int64_t or_pair_stream_text(const double *raw_xyz, int n, float rc, const int nc
 enum {
     OR_MODE_PARITY  = 1,   /* unfused arithmetic and dx = (xi - xj) - shift (reference order) */
     OR_MODE_COULOMB = 2,   /* add the r^-3 term */
-    OR_MODE_ANALYTIC = 4,  /* FP32 direct r^-14/r^-8/r^-3 instead of the tables */
+    OR_MODE_ANALYTIC = 4,  /* FP32 direct r^-14/r^-8/r^-3 instead of the tables, on the
+                              correctly rounded separation fl((x_i - t) - e) (t = fl(x_j + S)
+                              and e its exact error), the cutoff decided in FP64 */
 };
 
 typedef struct {
@@ -130,6 +132,7 @@ typedef struct {
     int mode;
     const uint32_t *excl;     /* optional exclusion windows, one per atom (NULL = none):
                                  bit d of excl[i] excludes the pair (i, i + d - 16) */
+    float rc;                 /* cutoff; analytic mode decides r^2 <= rc*rc exactly (FP64) */
 } or_rl_system;
 
 /* Cell index of one coordinate: (int)(x * inv_edge) clamped to [0, nc-1], with

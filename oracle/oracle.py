@@ -35,7 +35,8 @@ class RLSystem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("box", ctypes.c_float * 3), ("ncell", ctypes.c_int * 3),
                 ("rc2", ctypes.c_float), ("pos4", F), ("type", I), ("ntypes", ctypes.c_int),
                 ("pair_a", F), ("pair_b", F), ("tab", F), ("exp_offset", ctypes.c_int),
-                ("nseg", ctypes.c_int), ("mode", ctypes.c_int), ("excl", ctypes.c_void_p)]
+                ("nseg", ctypes.c_int), ("mode", ctypes.c_int), ("excl", ctypes.c_void_p),
+                ("rc", ctypes.c_float)]
 
 
 def build():
@@ -248,6 +249,7 @@ class RL:
         s.box[:] = [float(np.float32(b)) for b in box]
         s.ncell[:] = list(ncell)
         s.rc2 = float(np.float32(rc) * np.float32(rc))
+        s.rc = float(np.float32(rc))
         s.pos4, s.type = _p(self.pos4, F), _p(self.type, I)
         s.ntypes = len(eps)
         s.pair_a, s.pair_b, s.tab = _p(self.pa, F), _p(self.pb, F), _p(self.tab, F)

@@ -332,6 +332,22 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     s.n = atom_off;
     s.ncells = cell_off;
     c->p.n = s.n;
+    {
+        // Analytic mode's exact cutoff.  With positions in [0, L) and image shifts S in
+        // {-L, 0, L}, the staged fl(x_j + S) and dx = fl(x_i - fl(x_j + S)) are within
+        // u (2L + |D|) of the exact D (u = 2^-24), and the fused r2 within 3u r2 of |dx|^2,
+        // so |r2 - |D|^2| <= 3u rc^2 + 2 sqrt(3) u (2L + rc) rc near the cutoff; M bounds
+        // that with room.  The scan passes r2 <= rc^2 + M, and evaluation decides every
+        // pair above rc^2 - M from the exact FP64 r^2.
+        double lmax = 0.0;
+        for (const SysGeom &G : s.h_geom)
+            for (int d = 0; d < 3; ++d) lmax = std::max(lmax, (double)G.box[d]);
+        const double rcd = (double)p->rc, u = ldexp(1.0, -24);
+        const double m = 4.0 * u * (rcd * rcd + sqrt(3.0) * rcd * (2.0 * lmax + rcd));
+        s.rc2_64 = rcd * rcd;
+        s.rc2_hi = nextafterf((float)(s.rc2_64 + m), INFINITY);
+        s.rc2_lo = nextafterf((float)(s.rc2_64 - m), 0.0f);
+    }
     s.ntypes = p->ntypes;
     s.flags = p->flags;
     const md_rl_table *t = p->table;
@@ -1004,6 +1020,22 @@ extern "C" int md_rl_step_async(md_rl_ctx *c, int nsteps, void *hip_stream)
     }
     for (; k < nsteps; ++k)
         if ((rc = enqueue_step(c, st, c->timing))) return rc;
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_graph_prepare(md_rl_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) { set_error("md_rl_graph_prepare: bad argument"); return MDRL_EINVAL; }
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
+    // the graphs md_rl_step_async replays: 16-step ones while 16 steps remain, then 2-step
+    // ones; all at the parity the state has now (an even count returns it there)
+    if (!c->use_graph || c->timing || c->s.first_step) return MDRL_OK;
+    if (nsteps >= kGraphLong && !c->graph_long[c->s.parity] &&
+        (rc = capture_steps(c, kGraphLong, c->graph_long)))
+        return rc;
+    if (nsteps % kGraphLong >= 2 && !c->graph[c->s.parity] && (rc = capture_steps(c, 2, c->graph)))
+        return rc;
     return MDRL_OK;
 }
 

@@ -89,6 +89,10 @@ struct DeviceSystem {
     int n = 0;                                // atoms over all systems
     int ncells = 0;                           // cells over all systems
     float rc2 = 0.f;
+    // analytic mode's exact cutoff: the scan passes r2 <= rc2_hi (FP32); pairs with
+    // r2 > rc2_lo are then decided by the exact FP64 r^2 <= rc2_64 = rc * rc
+    float rc2_hi = 0.f, rc2_lo = 0.f;
+    double rc2_64 = 0.0;
     int ntypes = 1;
     int flags = 0;
     bool coul = false;

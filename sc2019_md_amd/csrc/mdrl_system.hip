@@ -329,7 +329,9 @@ struct ForceArgs {
     const int *cell_sys;
     const int *tasks;         // task order (persistent mode); NULL: task = blockIdx
     int ntasks;
-    float rc2;
+    float rc2;                // the scan's FP32 cutoff test r2 <= rc2
+    float rc2_lo;             // analytic mode: r2 above this is decided in FP64
+    double rc2_64;            // analytic mode: the exact test r^2 <= rc * rc
     const float *tab;
     int off, nseg, win_lo, nwin;
     const float2 *pair_ab;
@@ -343,6 +345,16 @@ struct ForceArgs {
     const uint32_t *excl;     // exclusion windows by input index, or NULL
     BondedArgs bd;            // bonded term blocks, tasks ntasks .. ntasks + bd.nblocks - 1
 };
+
+// The exact rounding error of the staged image fl(x + S) per component (Fast2Sum with
+// |S| >= |x| or S = 0): e = (x + S) - fl(x + S).  Zero where the sum is exact, which
+// includes S = -L for x in [L/2, L).
+__device__ __forceinline__ float4 shift_error(float4 x, float4 S)
+{
+    const float tx = x.x + S.x, ty = x.y + S.y, tz = x.z + S.z;
+    const float zx = tx - S.x, zy = ty - S.y, zz = tz - S.z;
+    return make_float4(x.x - zx, x.y - zy, x.z - zz, 0.0f);
+}
 
 // EXC: nonbonded exclusions; a queued pair whose neighbour lies in the home atom's
 // exclusion window contributes no force and is not counted.
@@ -358,6 +370,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float4 s_j[TJ];
     __shared__ int s_it[TI];
     __shared__ float s_jq[TJ];               // neighbour-tile charges (s_j[k].w holds k's queue bits)
+    __shared__ float4 s_je[ANA ? TJ : 1];    // analytic mode: rounding error of each staged fl(x + S)
     __shared__ unsigned long long s_iacc[3 * TI];
     __shared__ unsigned long long s_jacc[3 * TJ];
     __shared__ unsigned s_q[NWAVE][QCAP];
@@ -497,6 +510,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 float4 sh;
                 const int sl = locate(cur, tid, sh);
                 float4 p = a.pos[sl];
+                if (ANA) s_je[tid] = shift_error(p, sh);
                 if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
                 s_jq[tid] = p.w;
                 p.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)a.type[sl] << 16));
@@ -531,12 +545,35 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     const int ia = (int)(e & 0xffu), jb = (int)((e >> 8) & 0xffu);
                     const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
-                    const float r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
+                    float4 je;
+                    float r2;
+                    if (ANA) {
+                        // dx = fl((x_i - t) - e) with t = fl(x_j + S) and e its exact error:
+                        // the correctly rounded separation (x_i - t is exact across a face)
+                        je = s_je[jb];
+                        dx = (pa.x - pb.x) - je.x;
+                        dy = (pa.y - pb.y) - je.y;
+                        dz = (pa.z - pb.z) - je.z;
+                        r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    } else {
+                        r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
+                    }
                     const float2 ab = s_ab[e >> 16];
                     const float qq = pa.w * s_jq[jb];
                     float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
                                                                a.off, a.nseg, a.win_lo, a.nwin);
                     bool drop = r2 == 0.0f;
+                    if (ANA && r2 > a.rc2_lo) {
+                        // within the FP32 error margin of rc^2: the exact separation
+                        // D = (x_i - t) - e in FP64 (both differences exact there), and the
+                        // exact test |D|^2 <= rc^2
+                        const double Dx = ((double)pa.x - (double)pb.x) - (double)je.x;
+                        const double Dy = ((double)pa.y - (double)pb.y) - (double)je.y;
+                        const double Dz = ((double)pa.z - (double)pb.z) - (double)je.z;
+                        double e2 = Dx * Dx + Dy * Dy;
+                        e2 = e2 + Dz * Dz;
+                        drop = drop || !(e2 <= a.rc2_64);
+                    }
                     if (EXC) {
                         const int2 om = s_iom[ia];
                         const int d = s_jo[jb] - om.x + 16;
@@ -658,6 +695,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 if (tid < nxt.mj) {
                     float4 sh;   // slot and shift again from LDS rather than held in registers
                     const int sl = locate(nxt, tid, sh);
+                    if (ANA) s_je[tid] = shift_error(pf, sh);
                     if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
                     s_jq[tid] = pf.w;
                     pf.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)tf << 16));
@@ -791,7 +829,9 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
     a.geom = s.geom; a.cell_sys = s.cell_sys;
     a.tasks = s.sched.persistent ? s.tasks : nullptr;
-    a.rc2 = s.rc2;
+    a.rc2 = ana ? s.rc2_hi : s.rc2;
+    a.rc2_lo = s.rc2_lo;
+    a.rc2_64 = s.rc2_64;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
     a.win_lo = s.win.lo; a.nwin = s.win.n;
     a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;

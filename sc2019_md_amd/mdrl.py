@@ -165,6 +165,7 @@ def lib():
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]
         L.md_rl_set_graph.argtypes = [P, I]
+        L.md_rl_graph_prepare.argtypes = [P, I]
         L.md_rl_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
         L.md_rl_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
         L.md_rl_autotune.argtypes = [P, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_double)]
@@ -467,6 +468,10 @@ class System:
 
     def set_graph(self, on=True):
         _check(lib().md_rl_set_graph(self.handle, 1 if on else 0))
+
+    def prepare_graphs(self, nsteps):
+        """Capture (without running) the graphs that step(nsteps) would replay."""
+        _check(lib().md_rl_graph_prepare(self.handle, nsteps))
 
     def set_schedule(self, apw=64, tile=256, groups=1, persistent=1):
         sc = Schedule(apw, tile, groups, persistent)

@@ -191,3 +191,23 @@ def test_batch_with_empty_and_tiny_systems(apoa1):
     f2 = b.system_forces_fixed(3)
     assert np.all(f2[0] == -f2[1]) and np.any(f2 != 0)
     b.close()
+
+
+def test_prepared_graphs_bit_identical():
+    """md_rl_graph_prepare captures the graphs of the next 37 steps without running them
+    (the state is unchanged), and the steps that then replay them give the eager bits."""
+    s = mdrl.SyntheticSystem(1)
+    a = mdrl.System.from_synthetic(s)
+    a.step(38)
+    want = a.particles()
+    a.close()
+    b = mdrl.System.from_synthetic(s)
+    b.set_graph(True)
+    b.step(1)
+    before = b.particles()
+    b.prepare_graphs(37)          # one 16-step graph and one 2-step graph
+    assert np.array_equal(b.particles().view(np.uint32), before.view(np.uint32))
+    b.step(37)                    # two 16-step replays, two 2-step replays, one eager step
+    got = b.particles()
+    b.close()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

@@ -15,7 +15,8 @@ from sc2019_md_amd import mdrl
 pytestmark = pytest.mark.gpu
 
 from tests.test_oracle_golden import (  # noqa: E402
-    anisotropic_system, apoa1_tab6, cmodel_inputs, hex2f, parse_pdb_float32,
+    anisotropic_system, apoa1_tab6, cmodel_inputs, cutoff_edge_pairs, face_and_interior_pairs,
+    hex2f, parse_pdb_float32,
     parse_reference_output, pdb_text, rl_from_synthetic)
 
 
@@ -496,3 +497,62 @@ def test_non_finite_states_reported(apoa1):
     assert e.value.code == -6
     assert g.stats().flags & mdrl.F_NONFINITE
     g.close()
+
+
+def test_analytic_cutoff_decided_exactly(apoa1):
+    """400 two-atom systems within 2e-7 of rc across a periodic face, as one analytic-mode
+    batch: each pair is kept iff its exact FP64 r^2 <= rc^2, so the per-system int64 forces
+    equal the oracle's analytic mode and the pair count is the exact one.  The FP32 filter
+    of the table path would decide some of them the other way."""
+    base = mdrl.SyntheticSystem(4, 3)
+    L, rc = 60.0, 10.0
+    cases = cutoff_edge_pairs(400, L=L, rc=rc)
+    t = np.array([0, 1], np.int32)
+    q = np.array([0.3, -0.3], np.float32)
+    b = mdrl.Batch([2] * len(cases), [[L] * 3] * len(cases), rc, base.eps, base.sigma, base.mass,
+                   coulomb_k=base.coulomb_k, flags=mdrl.EVAL_ANALYTIC)
+    for k, (x, _) in enumerate(cases):
+        p = np.zeros((2, 11), np.float32)
+        p[:, :3] = x
+        b.set_system(k, p, t, q)
+    b.commit()
+    inside = 0
+    for k, (x, exact) in enumerate(cases):
+        want, n = O.RL(x, np.full(3, L, np.float32), [int(L // rc)] * 3, rc, t, base.eps,
+                       base.sigma, apoa1_tab6(apoa1), charge=q, coulomb_k=base.coulomb_k,
+                       mode=O.MODE_ANALYTIC).forces_fix()
+        assert n == int(exact)
+        assert np.array_equal(b.system_forces_fixed(k), want), "system %d" % k
+        inside += n
+    say("edge pairs: %d of %d within rc exactly" % (inside, len(cases)))
+    assert b.stats().pair_evals == inside
+    b.close()
+
+
+def test_analytic_separation_exact_across_the_periodic_face(apoa1):
+    """Pairs across a periodic face (where the staged image fl(x_j + L) rounds) and the same
+    exact separations inside the box, as one analytic-mode batch: every system's int64
+    forces equal the oracle's, and each face pair's forces equal its interior copy's."""
+    base = mdrl.SyntheticSystem(4, 3)
+    L, rc = 60.0, 12.0
+    cases = face_and_interior_pairs(100, L=L)
+    systems = [x for pair in cases for x in pair]
+    t = np.array([1, 2], np.int32)
+    q = np.array([0.3, -0.3], np.float32)
+    b = mdrl.Batch([2] * len(systems), [[L] * 3] * len(systems), rc, base.eps, base.sigma,
+                   base.mass, coulomb_k=base.coulomb_k, flags=mdrl.EVAL_ANALYTIC)
+    for k, x in enumerate(systems):
+        p = np.zeros((2, 11), np.float32)
+        p[:, :3] = x
+        b.set_system(k, p, t, q)
+    b.commit()
+    got = [b.system_forces_fixed(k) for k in range(len(systems))]
+    assert b.stats().pair_evals == len(systems)
+    b.close()
+    for k, x in enumerate(systems):
+        want, n = O.RL(x, np.full(3, L, np.float32), [5] * 3, rc, t, base.eps, base.sigma,
+                       apoa1_tab6(apoa1), charge=q, coulomb_k=base.coulomb_k,
+                       mode=O.MODE_ANALYTIC).forces_fix()
+        assert n == 1 and np.array_equal(got[k], want), "system %d" % k
+    for k in range(len(cases)):
+        assert np.array_equal(got[2 * k], got[2 * k + 1]), "pair %d: face != interior" % k

@@ -231,6 +231,110 @@ def anisotropic_system():
     return s
 
 
+def cutoff_edge_pairs(count, seed=11, L=60.0, rc=10.0):
+    """Two-atom systems whose separation lies within 2e-7 relative of rc, across a periodic
+    face (the staged image shift rounds there): positions as float32 rows in [0, L) and
+    whether each pair lies within rc exactly (FP64 of the float32 positions, minimum
+    image)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        a = k % 3
+        xi = rng.uniform(0.3 * L, 0.7 * L, 3)
+        xi[a] = L - rng.uniform(0.0, 0.5 * rc)
+        u = rng.normal(size=3)
+        u[a] = abs(u[a]) + 1.0
+        u /= np.linalg.norm(u)
+        xj = xi + rc * (1.0 + rng.uniform(-2e-7, 2e-7)) * u
+        x = np.stack([xi, xj]).astype(np.float32) % np.float32(L)
+        x = np.where(x >= np.float32(L), np.float32(0.0), x).astype(np.float32)
+        d = x[0].astype(np.float64) - x[1].astype(np.float64)
+        d -= L * np.round(d / L)
+        out.append((x, bool((d * d).sum() <= np.float64(np.float32(rc)) ** 2)))
+    return out
+
+
+def test_analytic_mode_decides_the_cutoff_exactly(apoa1):
+    """The analytic mode keeps a pair iff its exact FP64 r^2 <= rc^2, as the FP64 direct
+    evaluation does; the table path keeps the reference's FP32 test r2 <= rc2
+    (D/Filter_Logic.v:234), which differs on some of these pairs across the periodic face."""
+    base = mdrl.SyntheticSystem(4, 3)
+    L, rc = 60.0, 10.0
+    flips = 0
+    for x, inside in cutoff_edge_pairs(400, L=L, rc=rc):
+        t = np.array([0, 1], np.int32)
+        q = np.array([0.3, -0.3], np.float32)
+        box = np.full(3, L, np.float32)
+        ncell = [int(L // rc)] * 3
+        kw = dict(charge=q, coulomb_k=base.coulomb_k)
+        tab = apoa1_tab6(apoa1)
+        _, n_ana = O.RL(x, box, ncell, rc, t, base.eps, base.sigma, tab, mode=O.MODE_ANALYTIC,
+                        **kw).forces_fix()
+        _, n_tab = O.RL(x, box, ncell, rc, t, base.eps, base.sigma, tab, **kw).forces_fix()
+        _, _, _, n64 = O.fp64_direct(x, t, base.eps, base.sigma, box, rc, charge=q,
+                                     ke=base.coulomb_k, use_cells=False)
+        assert n_ana == int(inside) == n64
+        flips += n_tab != n_ana
+    assert flips >= 8, "the sample should hold pairs the FP32 test decides the other way"
+
+
+def face_and_interior_pairs(count, seed=5, L=60.0):
+    """Pairs of two-atom systems (box L = 60 A, rc 12 A) with the same exact separation D:
+    one across the periodic face of an axis, where x_j + L lands in [64, 72) and rounds (x_j
+    an odd multiple of 2^-18), one inside the box (S = 0, every difference exact).  All
+    coordinates and D are multiples of 2^-18, so D is exact in FP32."""
+    rng = np.random.default_rng(seed)
+    q = 2.0 ** -18
+    out = []
+    for k in range(count):
+        a = k % 3
+        xj_face = 4.0 + (2 * int(rng.integers(0, 2 ** 17)) + 1) * q      # in [4, 5)
+        xi_face = 56.0 + int(rng.integers(0, 3 * 2 ** 18)) * q              # in [56, 59)
+        D = np.zeros(3)
+        D[a] = xi_face - xj_face - L                                        # in (-9, -5)
+        if D[a] < -10.0 or D[a] > -2.0:
+            continue
+        for b in range(3):
+            if b != a:
+                D[b] = int(rng.integers(-3 * 2 ** 18, 3 * 2 ** 18)) * q
+        face = np.full((2, 3), 30.0)
+        face[0] = 30.0 + D
+        face[0, a], face[1, a] = xi_face, xj_face
+        inner = np.full((2, 3), 30.0)
+        inner[0] = 30.0 + D
+        out.append((face.astype(np.float32), inner.astype(np.float32)))
+    return out
+
+
+def test_analytic_separation_exact_across_the_periodic_face(apoa1):
+    """The analytic mode evaluates a pair across a periodic face on its exact separation:
+    the int64 forces equal those of the same separation inside the box, bit for bit.  The
+    table path's pre-shifted image fl(x_j + L) carries the rounding (its forces differ on
+    some of these pairs, which shows the comparison can fail)."""
+    base = mdrl.SyntheticSystem(4, 3)
+    L, rc = 60.0, 12.0
+    box = np.full(3, L, np.float32)
+    t = np.array([1, 2], np.int32)
+    q = np.array([0.3, -0.3], np.float32)
+    tab = apoa1_tab6(apoa1)
+    cases = face_and_interior_pairs(100, L=L)
+    assert len(cases) > 50
+    differ_tab = 0
+    for face, inner in cases:
+        got = []
+        for mode in (O.MODE_ANALYTIC, 0):
+            ff, nf = O.RL(face, box, [5] * 3, rc, t, base.eps, base.sigma, tab, charge=q,
+                          coulomb_k=base.coulomb_k, mode=mode).forces_fix()
+            fi, ni = O.RL(inner, box, [5] * 3, rc, t, base.eps, base.sigma, tab, charge=q,
+                          coulomb_k=base.coulomb_k, mode=mode).forces_fix()
+            assert nf == ni == 1
+            got.append((ff, fi))
+        (fa, fai), (ft, fti) = got
+        assert np.array_equal(fa, fai), (face, inner)
+        differ_tab += not np.array_equal(ft, fti)
+    assert differ_tab >= 5
+
+
 def test_half_shell_newton_third_law_exact(apoa1):
     s = mdrl.SyntheticSystem(1)
     rl = rl_from_synthetic(s, apoa1)
