@@ -62,6 +62,33 @@ __device__ __forceinline__ void fix3(float fx, float fy, float fz, long long &kx
     }
 }
 
+// fix3 of (fx, fy, fz) given as fx * 2^32 etc. (the products of a force scaled by 2^32,
+// which carry the same bits times 2^32): the fast path is one FP64 add of 1.5 * 2^52
+// (rounding to the integer, ties to even) and the integer's bits; saturation at +-2^62
+// and NaN as fix_of.
+__device__ __forceinline__ long long fix_scaled_slow(float x, unsigned &flag)
+{
+    double d = (double)x;
+    if (d != d) { flag |= 2u; return 0; }
+    if (d > 0x1p62) { d = 0x1p62; flag |= 1u; }
+    if (d < -0x1p62) { d = -0x1p62; flag |= 1u; }
+    return (long long)rint(d);
+}
+__device__ __forceinline__ void fix3_scaled(float x, float y, float z, long long &kx, long long &ky,
+                                            long long &kz, unsigned &flag)
+{
+    const float m = (fabsf(x) + fabsf(y)) + fabsf(z);
+    if (m < 0x1p51f) {
+        kx = __double_as_longlong((double)x + 6755399441055744.0) - 0x4338000000000000LL;
+        ky = __double_as_longlong((double)y + 6755399441055744.0) - 0x4338000000000000LL;
+        kz = __double_as_longlong((double)z + 6755399441055744.0) - 0x4338000000000000LL;
+    } else {
+        kx = fix_scaled_slow(x, flag);
+        ky = fix_scaled_slow(y, flag);
+        kz = fix_scaled_slow(z, flag);
+    }
+}
+
 __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
 {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));

@@ -93,6 +93,9 @@ struct DeviceSystem {
     // r2 > rc2_lo are then decided by the exact FP64 r^2 <= rc2_64 = rc * rc
     float rc2_hi = 0.f, rc2_lo = 0.f;
     double rc2_64 = 0.0;
+    // the candidate scan's threshold on its |p_i|^2 + |p_j|^2 - 2 p_i.p_j (corner-relative
+    // coordinates): the cutoff of the mode plus the bound on that r2's error
+    float rc2_scan = 0.f, rc2_scan_ana = 0.f;
     int ntypes = 1;
     int flags = 0;
     bool coul = false;

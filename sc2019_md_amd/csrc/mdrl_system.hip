@@ -329,7 +329,8 @@ struct ForceArgs {
     const int *cell_sys;
     const int *tasks;         // task order (persistent mode); NULL: task = blockIdx
     int ntasks;
-    float rc2;                // the scan's FP32 cutoff test r2 <= rc2
+    float rc2;                // the reference's FP32 cutoff test r2 <= rc2
+    float rc2_scan;           // the candidate scan's test on its own r2 (rc2 plus its error bound)
     float rc2_lo;             // analytic mode: r2 above this is decided in FP64
     double rc2_64;            // analytic mode: the exact test r^2 <= rc * rc
     const float *tab;
@@ -340,6 +341,7 @@ struct ForceArgs {
     float *f32;
     unsigned long long *counters;
     int ngroups;
+    int progress_prio;        // set wave priorities from task progress (few tasks per workgroup)
     int gstart[8];
     const int *orig;          // slot -> input index (exclusions)
     const uint32_t *excl;     // exclusion windows by input index, or NULL
@@ -369,7 +371,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float4 s_i[TI];
     __shared__ float4 s_j[TJ];
     __shared__ int s_it[TI];
-    __shared__ float s_jq[TJ];               // neighbour-tile charges (s_j[k].w holds k's queue bits)
+    // neighbour tile for the candidate scan: image position relative to the home cell's
+    // corner, .w = the atom's queue bits; entries [mj, mj + 64) repeat [0, 64), so the
+    // rotated index of a step never wraps
+    __shared__ float4 s_js[TJ + 64];
     __shared__ float4 s_je[ANA ? TJ : 1];    // analytic mode: rounding error of each staged fl(x + S)
     __shared__ unsigned long long s_iacc[3 * TI];
     __shared__ unsigned long long s_jacc[3 * TJ];
@@ -404,7 +409,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     unsigned *q = s_q[wave];
     unsigned flag = 0;
     unsigned long long npairs = 0, ncand = 0, ndrop = 0;
-    const float rc2 = a.rc2;
+    const float rc2s = a.rc2_scan;
 
     // One block of bonded terms: forces accumulate in the home-tile accumulators
     // (3 x kBondedBlockSpan words: fixed point, or FP32 in FP32-atomics mode) and reach HBM
@@ -449,6 +454,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
         const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
         const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
         const int obeg = a.gstart[grp], oend = a.gstart[grp + 1];
+        // the home cell's corner: origin of the scan's relative coordinates
+        const float x0 = (float)hx * (G.box[0] / (float)nc0), y0 = (float)hy * (G.box[1] / (float)nc1),
+                    z0 = (float)hz * (G.box[2] / (float)nc2);
+        // the scan's record of a staged neighbour: image relative to the corner, queue bits
+        auto scan_rec = [&](float4 p, float4 sh, unsigned bits) {
+            return make_float4((p.x + sh.x) - x0, (p.y + sh.y) - y0, (p.z + sh.z) - z0, __uint_as_float(bits));
+        };
 
         // the task's cells: first slot, count and image shift, one thread per offset
         if (tid >= obeg && tid < oend) {
@@ -511,10 +523,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const int sl = locate(cur, tid, sh);
                 float4 p = a.pos[sl];
                 if (ANA) s_je[tid] = shift_error(p, sh);
+                const float4 r = scan_rec(p, sh, ((unsigned)tid << 8) + ((unsigned)a.type[sl] << 16));
+                s_js[tid] = r;
+                if (tid < 64) s_js[cur.mj + tid] = r;
                 if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
-                s_jq[tid] = p.w;
-                p.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)a.type[sl] << 16));
-                s_j[tid] = p;
+                s_j[tid] = p;                   // .w: the scaled charge
                 s_jslot[0][tid] = sl;
                 if (EXC) s_jo[tid] = a.orig[sl];
             }
@@ -523,6 +536,20 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             int jpar = 0;
 
             while (cur.o < oend) {
+                // Issue priority from the task's progress (3 in its first quarter of offsets,
+                // 0 in its last), when workgroups hold one or two tasks each: the SIMDs issue
+                // to the oldest ready wave among equal priorities, so the workgroup dispatched
+                // first to a CU would otherwise run ahead and leave its partner to finish
+                // alone; this keeps them in step.  (With many tasks per workgroup the queue
+                // balances the load and the priorities only disturb it.)
+                if (a.progress_prio) {
+                    switch ((4 * (cur.o - obeg)) / (oend - obeg)) {
+                    case 0: __builtin_amdgcn_s_setprio(3); break;
+                    case 1: __builtin_amdgcn_s_setprio(2); break;
+                    case 2: __builtin_amdgcn_s_setprio(1); break;
+                    default: __builtin_amdgcn_s_setprio(0); break;
+                    }
+                }
                 const NbrTile nxt = (cur.oe == cur.o + 1) ? find(cur.o, cur.jt + TJ) : find(cur.oe, 0);
                 float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
                 int tf = 0, of = 0;
@@ -559,10 +586,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
                     }
                     const float2 ab = s_ab[e >> 16];
-                    const float qq = pa.w * s_jq[jb];
+                    const float qq = pa.w * pb.w;
                     float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
                                                                a.off, a.nseg, a.win_lo, a.nwin);
-                    bool drop = r2 == 0.0f;
+                    // the reference filter 0 < r2 <= rc2 on the pair's own r2 (the scan passed
+                    // a superset): one unsigned compare of the bit patterns (r2 >= 0)
+                    bool drop = ANA ? (r2 == 0.0f) : (__float_as_uint(r2) - 1u >= __float_as_uint(a.rc2));
                     if (ANA && r2 > a.rc2_lo) {
                         // within the FP32 error margin of rc^2: the exact separation
                         // D = (x_i - t) - e in FP64 (both differences exact there), and the
@@ -581,8 +610,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     }
                     ndrop += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(drop));
                     F = drop ? 0.0f : F;
-                    const float fx = F * dx, fy = F * dy, fz = F * dz;
                     if (F32) {
+                        const float fx = F * dx, fy = F * dy, fz = F * dz;
                         atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
                         atomic_add_wg(&s_iacc_f[3 * ia + 1], fy);
                         atomic_add_wg(&s_iacc_f[3 * ia + 2], fz);
@@ -590,8 +619,16 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
                         atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
                     } else {
+                        // rint(F*d * 2^32): F is scaled first (exact, and the products then
+                        // carry the bits of F*d times 2^32); a force too large to scale, or
+                        // not finite, takes the unscaled conversion
                         long long kx, ky, kz;
-                        fix3(fx, fy, fz, kx, ky, kz, flag);
+                        if (fabsf(F) < 0x1p95f) {
+                            const float Fk = F * 4294967296.0f;
+                            fix3_scaled(Fk * dx, Fk * dy, Fk * dz, kx, ky, kz, flag);
+                        } else {
+                            fix3(F * dx, F * dy, F * dz, kx, ky, kz, flag);
+                        }
                         atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
                         atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
                         atomic_add_wg(&s_iacc[3 * ia + 2], (unsigned long long)kz);
@@ -611,12 +648,23 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     // this lane's home atoms (far away past the tile: r2 = inf never
                     // passes) and, in the home cell, the neighbour index each pair must
                     // exceed so that a pair is taken once as (lower, higher)
-                    float4 pi[NCHUNK];
+                    // The scan's r2 = (c_i + c_j) - 2 p_i . p_j on corner-relative coordinates
+                    // (c = |p|^2): each home atom holds -2 p_i and c_i, lanes past the tile a
+                    // c_i that never passes.  It differs from the reference r2 by less than the
+                    // margin folded into rc2_scan; evaluation applies the reference test.
+                    float hx2[NCHUNK], hy2[NCHUNK], hz2[NCHUNK], hc[NCHUNK];
                     unsigned eb[NCHUNK];
 #pragma unroll
                     for (int ch = 0; ch < NCHUNK; ++ch) {
                         const int ia = cg0 + ch * APW + il;
-                        pi[ch] = (ia < mi) ? s_i[ia] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+                        if (ia < mi) {
+                            const float4 p = s_i[ia];
+                            const float xr = p.x - x0, yr = p.y - y0, zr = p.z - z0;
+                            hx2[ch] = -2.0f * xr; hy2[ch] = -2.0f * yr; hz2[ch] = -2.0f * zr;
+                            hc[ch] = fmaf(zr, zr, fmaf(yr, yr, xr * xr));
+                        } else {
+                            hx2[ch] = 0.0f; hy2[ch] = 0.0f; hz2[ch] = 0.0f; hc[ch] = 1e30f;
+                        }
                         eb[ch] = (unsigned)ia + ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << 16);
                     }
                     // SELF: the home cell, where a pair needs jb > itile + ia - jtile
@@ -627,17 +675,15 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         for (int sft = wave; sft < nsteps; sft += NWAVE) {
                             const int k = ph + sft * P;
                             const bool kv = (P == 1) || (k < mj);
-                            int jb = bj + k;
-                            jb = (jb >= mj) ? jb - mj : jb;
-                            jb = kv ? jb : 0;
-                            const float4 pj = s_j[jb];
+                            const float4 pj = s_js[kv ? bj + k : 0];   // bj + k < mj + 64
                             const unsigned jsh = __float_as_uint(pj.w);
+                            const int jb = (int)((jsh >> 8) & 0xffu);
+                            const float cj = fmaf(pj.z, pj.z, fmaf(pj.y, pj.y, pj.x * pj.x));
 #pragma unroll
                             for (int ch = 0; ch < NCHUNK; ++ch) {
                                 if (ch < nch) {
-                                    float dx, dy, dz;
-                                    const float r2 = pair_geom<PAR>(pi[ch], pj, sx, sy, sz, dx, dy, dz);
-                                    const bool pass = kv && (r2 <= rc2) && (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
+                                    const float r2 = fmaf(hz2[ch], pj.z, fmaf(hy2[ch], pj.y, fmaf(hx2[ch], pj.x, hc[ch] + cj)));
+                                    const bool pass = kv && (r2 <= rc2s) && (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
                                     const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
                                     // slot = tail + the lane's rank among the passing lanes
                                     if (pass)
@@ -696,9 +742,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     float4 sh;   // slot and shift again from LDS rather than held in registers
                     const int sl = locate(nxt, tid, sh);
                     if (ANA) s_je[tid] = shift_error(pf, sh);
+                    const float4 r = scan_rec(pf, sh, ((unsigned)tid << 8) + ((unsigned)tf << 16));
+                    s_js[tid] = r;
+                    if (tid < 64) s_js[nxt.mj + tid] = r;
                     if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
-                    s_jq[tid] = pf.w;
-                    pf.w = __uint_as_float(((unsigned)tid << 8) + ((unsigned)tf << 16));
                     s_j[tid] = pf;
                     s_jslot[jpar ^ 1][tid] = sl;
                     if (EXC) s_jo[tid] = of;
@@ -795,8 +842,30 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
 }
 
 template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false>
-static void launch_rl(const ForceArgs &a, dim3 grid, size_t lds, hipStream_t st)
+static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st)
 {
+    // Two workgroups share a CU's 160 KiB of LDS only while each stays within 80 KiB: the
+    // table window keeps its top (the bins just below rc^2, where most pairs fall) and is cut
+    // from below to what fits beside this variant's static arrays; entries below it are read
+    // from the global table, which gives the same bits.
+    static size_t static_lds = ~(size_t)0;
+    if (static_lds == ~(size_t)0) {
+        hipFuncAttributes fa{};
+        static_lds = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(
+                         &k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>)) == hipSuccess
+                         ? fa.sharedSizeBytes : 0;
+    }
+    ForceArgs a = a0;
+    constexpr size_t kHalfCu = 80 * 1024 - 512;   // half the CU's LDS, less an allocation granule
+    const size_t stride = (COUL ? 6 : 4) * sizeof(float);
+    if (lds > 0 && static_lds > 0 && static_lds + lds > kHalfCu) {
+        const int fit = static_lds < kHalfCu ? (int)((kHalfCu - static_lds) / stride) : 0;
+        if (fit < a.nwin) {
+            a.win_lo += a.nwin - fit;
+            a.nwin = fit;
+            lds = (size_t)fit * stride;
+        }
+    }
     hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>), grid, dim3(NTHR), lds, st, a);
 }
 
@@ -829,7 +898,8 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
     a.geom = s.geom; a.cell_sys = s.cell_sys;
     a.tasks = s.sched.persistent ? s.tasks : nullptr;
-    a.rc2 = ana ? s.rc2_hi : s.rc2;
+    a.rc2 = s.rc2;
+    a.rc2_scan = ana ? s.rc2_scan_ana : s.rc2_scan;
     a.rc2_lo = s.rc2_lo;
     a.rc2_64 = s.rc2_64;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
@@ -844,6 +914,7 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     const size_t lds = ana ? 0 : (size_t)s.win.n * str * sizeof(float);
     const int ntot = a.ntasks + a.bd.nblocks;
     const int grid_n = s.sched.persistent ? std::min(ntot, 2 * s.cu_count) : ntot;
+    a.progress_prio = ntot <= 2 * grid_n;
     const dim3 grid(grid_n);
     const Schedule &sc = s.sched;
     if (ana) {
