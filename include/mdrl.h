@@ -267,6 +267,22 @@ int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n);
  * kinetic energy of the full-step velocities and the pair virial (sum r_ij F_ij), each
  * reduced in a fixed order so repeated calls give identical bits. */
 int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[9]);
+/* The parts of md_rl_energy for system sys (-1: all systems), in kJ/mol.  cutoff_shift is
+ * the sum over the in-cutoff, non-excluded pairs of the table pair energy at rc: the
+ * reference's potentials are truncated without a shift (the *_no_smooth tables), so the
+ * potential jumps by a pair's energy at rc whenever the pair crosses the cutoff, and
+ * potential - cutoff_shift (the shifted potential, whose forces are the same) is what
+ * velocity Verlet conserves together with the kinetic energy.  No reference counterpart. */
+typedef struct md_rl_energy_terms {
+    double kinetic;            /* 0.5 m v^2 at full-step velocities */
+    double potential;          /* range_limited + bonded + long_range, as md_rl_energy */
+    double range_limited;      /* pair energies from the c*_{12,6,1} tables */
+    double bonded;
+    double long_range;         /* the summation hook's energy */
+    double virial[9];          /* sum over pairs of F_ij (x) r_ij, kJ/mol */
+    double cutoff_shift;
+} md_rl_energy_terms;
+int md_rl_get_energy_terms(md_rl_ctx *c, int sys, md_rl_energy_terms *out);
 int md_rl_get_stats(md_rl_ctx *c, md_rl_stats *s);
 /* Enables HIP-event timing of each force kernel and full step (off by default). */
 int md_rl_set_timing(md_rl_ctx *c, int on);

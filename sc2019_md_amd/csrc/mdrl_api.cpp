@@ -222,7 +222,7 @@ static void free_ctx_device(md_rl_ctx *c)
     void *ptrs[] = {s.geom, s.cell_sys, s.slot_sys, s.tasks, s.pos, s.pos2, s.vel, s.vel2, s.type,
                     s.type2, s.orig, s.orig2, s.cell, s.perm, s.cell_start, s.cell_next,
                     s.cell_fill, s.fix, s.f32, s.tab, s.etab, s.pair_ab, s.pair_ab_e, s.hk,
-                    s.mass, s.counters, s.cell_red, s.atom_ke, s.red, c->d_tmp4, s.excl,
+                    s.mass, s.counters, s.cell_red, s.ke_chunks, s.chunk_ke, s.red, c->d_tmp4, s.excl,
                     s.slot_of, s.term_atoms, s.term_params, s.term_e, s.term_off, s.term_blocks,
                     c->d_lr_xq, c->d_lr_f, c->d_lr_e, c->d_q};
     for (void *p : ptrs)
@@ -325,6 +325,9 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
         G.atom_off = atom_off;
         G.cell_off = cell_off;
         G.n = nat[r];
+        G.ke_off = s.nke;
+        G.ke_n = (nat[r] + kKeChunk - 1) / kKeChunk;
+        s.nke += G.ke_n;
         atom_off += nat[r];
         cell_off += G.nc[0] * G.nc[1] * G.nc[2];
         s.h_geom.push_back(G);
@@ -398,6 +401,18 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     }
     c->qscale = s.coul ? sqrt(10.0 * (double)p->coulomb_k * 4.0 * 3.14) : 0.0;
     {
+        // the energy tables at rc^2 (the entry the pair path would read there)
+        uint32_t u;
+        memcpy(&u, &s.rc2, 4);
+        int seg = (int)((u >> 23) & 0xff) - t->exp_offset;
+        if (seg < 0 || seg >= t->nseg) seg = 0;
+        const int idx = seg * 256 + (int)((u >> 15) & 0xff);
+        auto at_rc = [&](int k) { return t->has[k] ? fmaf(t->c1[k][idx], s.rc2, t->c0[k][idx]) : 0.0f; };
+        s.u12c = at_rc(MDRL_TERM_12);
+        s.u6c = at_rc(MDRL_TERM_6);
+        s.u1c = at_rc(MDRL_TERM_1) * 0.1f;
+    }
+    {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
@@ -430,8 +445,9 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
     if (!st) st = dalloc(&s.hk, nt);
     if (!st) st = dalloc(&s.mass, nt);
     if (!st) st = dalloc(&s.counters, kCounterWords);
-    if (!st) st = dalloc(&s.cell_red, 10 * (size_t)s.ncells);
-    if (!st) st = dalloc(&s.atom_ke, n);
+    if (!st) st = dalloc(&s.cell_red, 11 * (size_t)s.ncells);
+    if (!st) st = dalloc(&s.ke_chunks, s.nke);
+    if (!st) st = dalloc(&s.chunk_ke, s.nke);
     if (!st) st = dalloc(&s.red, (size_t)kRedStride * nsys);
     if (!st) st = dalloc(&c->d_tmp4, n);
     if (st) { md_rl_destroy(c); return st; }
@@ -471,6 +487,13 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
         return hip_fail(e, "hipMemcpy");                                                  \
     }
     UP(s.geom, s.h_geom.data(), sizeof(SysGeom) * nsys);
+    {
+        std::vector<int2> kc;
+        for (const SysGeom &G : s.h_geom)
+            for (int k = 0; k < G.ke_n; ++k)
+                kc.push_back(make_int2(G.atom_off + k * kKeChunk, std::min(kKeChunk, G.n - k * kKeChunk)));
+        if (!kc.empty()) UP(s.ke_chunks, kc.data(), sizeof(int2) * kc.size());
+    }
     if (s.ncells) UP(s.cell_sys, cell_sys.data(), sizeof(int) * s.ncells);
     if (n) UP(s.slot_sys, slot_sys.data(), sizeof(int) * n);
     UP(s.pair_ab, ab.data(), sizeof(float2) * ab.size());
@@ -701,7 +724,7 @@ static int pull_state(md_rl_ctx *c)
     std::vector<float4> p(n), v(n);
     std::vector<int> orig(n);
     int rc;
-    if ((rc = launch_energy(s, c->d_tmp4, c->stream))) return rc;
+    if ((rc = launch_energy(s, c->d_tmp4, c->stream, kEnergyKinetic))) return rc;
     hipError_t e;
     if ((e = hipMemcpyAsync(p.data(), s.pos, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(v.data(), c->d_tmp4, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
@@ -1102,7 +1125,7 @@ static int read_system(md_rl_ctx *c, int sys, float *particle, int64_t *fix)
     std::vector<long long> f(3 * (size_t)n);
     std::vector<float> g;
     int rc;
-    if (particle && (rc = launch_energy(s, c->d_tmp4, c->stream))) return rc;
+    if (particle && (rc = launch_energy(s, c->d_tmp4, c->stream, kEnergyKinetic))) return rc;
     hipError_t e = hipSuccess;
     const size_t o = G.atom_off;
     if (particle) {
@@ -1163,13 +1186,14 @@ extern "C" int md_rl_get_forces_fixed(md_rl_ctx *c, int64_t *fix, int n)
     return md_rl_get_system_forces_fixed(c, 0, fix);
 }
 
-// Per-system FP64 reductions (kRedStride each) and the summation hook's energies.
-static int energies(md_rl_ctx *c, std::vector<double> &red, std::vector<double> &lr)
+// Per-system FP64 reductions (kRedStride each) and the summation hook's energies; the pair
+// virial only when asked for (its slots hold the bonded part alone otherwise).
+static int energies(md_rl_ctx *c, std::vector<double> &red, std::vector<double> &lr, bool virial)
 {
     int rc = ensure_uploaded(c);
     if (rc) return rc;
     DeviceSystem &s = c->s;
-    if ((rc = launch_energy(s, nullptr, c->stream))) return rc;
+    if ((rc = launch_energy(s, nullptr, c->stream, virial ? kEnergyVirial : kEnergyPotential))) return rc;
     if ((rc = run_long_range(c, c->stream, true))) return rc;
     red.resize((size_t)kRedStride * s.nsys);
     lr.assign(s.nsys, 0.0);
@@ -1184,7 +1208,7 @@ extern "C" int md_rl_system_energy(md_rl_ctx *c, int sys, double *pe, double *ke
 {
     if (!c || sys < 0 || sys >= c->s.nsys) { set_error("md_rl_system_energy: bad argument"); return MDRL_EINVAL; }
     std::vector<double> red, lr;
-    int rc = energies(c, red, lr);
+    int rc = energies(c, red, lr, virial != nullptr);
     if (rc) return rc;
     const double *r = red.data() + (size_t)kRedStride * sys;
     if (ke) *ke = r[0];
@@ -1197,7 +1221,7 @@ extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[
 {
     if (!c) { set_error("md_rl_energy: bad argument"); return MDRL_EINVAL; }
     std::vector<double> red, lr;
-    int rc = energies(c, red, lr);
+    int rc = energies(c, red, lr, virial != nullptr);
     if (rc) return rc;
     double t[12] = {0}, tlr = 0.0;
     for (int r = 0; r < c->s.nsys; ++r) {
@@ -1207,6 +1231,28 @@ extern "C" int md_rl_energy(md_rl_ctx *c, double *pe, double *ke, double virial[
     if (ke) *ke = t[0];
     if (pe) *pe = t[1] + t[11] + tlr;
     if (virial) for (int k = 0; k < 9; ++k) virial[k] = t[2 + k];
+    return MDRL_OK;
+}
+
+extern "C" int md_rl_get_energy_terms(md_rl_ctx *c, int sys, md_rl_energy_terms *out)
+{
+    if (!c || !out || sys < -1 || sys >= c->s.nsys) { set_error("md_rl_get_energy_terms: bad argument"); return MDRL_EINVAL; }
+    std::vector<double> red, lr;
+    int rc = energies(c, red, lr, true);
+    if (rc) return rc;
+    double t[kRedStride] = {0}, tlr = 0.0;
+    for (int r = 0; r < c->s.nsys; ++r) {
+        if (sys >= 0 && r != sys) continue;
+        for (int k = 0; k < kRedStride; ++k) t[k] += red[(size_t)kRedStride * r + k];
+        tlr += lr[r];
+    }
+    out->kinetic = t[0];
+    out->range_limited = t[1];
+    out->bonded = t[11];
+    out->long_range = tlr;
+    out->potential = t[1] + t[11] + tlr;
+    for (int k = 0; k < 9; ++k) out->virial[k] = t[2 + k];
+    out->cutoff_shift = t[12];
     return MDRL_OK;
 }
 

@@ -52,7 +52,12 @@ struct SysGeom {
     int cell_off;
     int atom_off;
     int n;
+    int ke_off, ke_n;       // the system's kinetic-energy chunks (kKeChunk slots each)
 };
+
+// Kinetic energies are reduced in chunks of at most kKeChunk slots within one system (one
+// workgroup each, a fixed tree), then per system over its chunks.
+constexpr int kKeChunk = 2048;
 
 // Force-kernel schedule (autotuned): home atoms per wavefront, neighbour-tile depth,
 // half-shell offset groups per workgroup, persistent work queue.
@@ -81,7 +86,7 @@ constexpr int kCounterWords = kQueueWord + kQueues;
 // Copies of the per-cell counts of the binning (see count_copy in the kernels).
 constexpr int kCountCopies = 8;
 
-constexpr int kRedStride = 12;          // per-system FP64 reductions: KE, PE, virial[9], bonded PE
+constexpr int kRedStride = 13;          // per-system FP64 reductions: KE, PE, virial[9], bonded PE, PE at rc
 
 // Device state of a batch of systems.
 struct DeviceSystem {
@@ -96,6 +101,8 @@ struct DeviceSystem {
     // the candidate scan's threshold on its |p_i|^2 + |p_j|^2 - 2 p_i.p_j (corner-relative
     // coordinates): the cutoff of the mode plus the bound on that r2's error
     float rc2_scan = 0.f, rc2_scan_ana = 0.f;
+    // the energy tables' t12, t6 and t1 / 10 at rc^2: the pair energy at the cutoff
+    float u12c = 0.f, u6c = 0.f, u1c = 0.f;
     int ntypes = 1;
     int flags = 0;
     bool coul = false;
@@ -126,8 +133,10 @@ struct DeviceSystem {
     unsigned long long *counters = nullptr;   // kCounterWords: [2] flags [3] max cell, then
                                               // kStatSlots x {pairs, candidates}, then the
                                               // kQueues work counters
-    double *cell_red = nullptr;               // ncells x 10: PE + virial partials per home cell
-    double *atom_ke = nullptr;                // n: kinetic energy per slot
+    double *cell_red = nullptr;               // ncells x 11: PE, virial, PE at rc per home cell
+    int2 *ke_chunks = nullptr;                // nke: first slot, slot count of each chunk
+    double *chunk_ke = nullptr;               // nke: kinetic energy per chunk
+    int nke = 0;
     double *red = nullptr;                    // nsys x kRedStride
     // bonded terms and nonbonded exclusions (batch-wide input indices)
     uint32_t *excl = nullptr;                 // n exclusion windows by input index, or NULL
@@ -154,7 +163,10 @@ int launch_force(DeviceSystem &s, hipStream_t st);
 int launch_update(DeviceSystem &s, float dt, hipStream_t st);
 int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
 int launch_zero_forces(DeviceSystem &s, hipStream_t st);   // forces and pair counters
-int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st);   // red[] per system
+// Energy reductions red[] per system: kinetic only (full-step velocities into vout), the
+// potential energy and energy at rc, or those with the pair virial.
+enum { kEnergyKinetic = 0, kEnergyPotential = 1, kEnergyVirial = 2 };
+int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st, int mode);
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc);
 // Summation hook: {x, y, z, q} records in input order out of the slots (clearing the
 // provider's forces f, n x 3, and energies e, nsys), and the provider's forces back in.

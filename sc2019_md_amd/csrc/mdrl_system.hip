@@ -334,8 +334,12 @@ struct ForceArgs {
     float rc2_lo;             // analytic mode: r2 above this is decided in FP64
     double rc2_64;            // analytic mode: the exact test r^2 <= rc * rc
     const float *tab;
+    const float *etab;        // energy pass: c0/c1 of r^-12, r^-6, r^-1, 6 floats per entry
     int off, nseg, win_lo, nwin;
     const float2 *pair_ab;
+    const float2 *pair_ab_e;  // energy pass: (eps sig^12, eps sig^6) per type pair
+    float u12c, u6c, u1c;     // energy pass: t12, t6 and t1 / 10 at rc^2 (the energy at rc)
+    double *cell_red;         // energy pass: 11 sums per home cell
     int ntypes;
     long long *fix;
     float *f32;
@@ -360,10 +364,19 @@ __device__ __forceinline__ float4 shift_error(float4 x, float4 S)
 
 // EXC: nonbonded exclusions; a queued pair whose neighbour lies in the home atom's
 // exclusion window contributes no force and is not counted.
-template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false>
+// EN (1 or 2): the energy pass instead of forces: the same pair set, each pair's energy
+// from the c*_{12,6,1} tables (M/FullScale_Simulation_LJArgon.m:938-1010), its energy at rc
+// and (EN = 1) its virial F d (x) d from the force tables.  Per lane and home cell the
+// energies are summed in FP64 and the virial in FP32 (the lane's pairs of one cell, about
+// a hundred at 0.1 atoms/A^3 and rc 12 A); from there on in FP64: lanes, waves, then cells
+// in order.  One task per home cell and a fixed launch shape: bit-identical run to run.
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false, int EN = 0>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_rl_halfshell(ForceArgs a)
 {
-    constexpr int STR = COUL ? 6 : 4;
+    static_assert(!EN || (!F32 && !PAR && !ANA && APW == 64), "the energy pass uses the table path");
+    constexpr int FSTR = COUL ? 6 : 4;   // coefficients per table entry (force, or EN energy)
+    constexpr int STR = FSTR;            // floats per window entry
+    constexpr int NACC = EN == 1 ? 8 : 2;   // EN: E, E(rc)[, W xx xy xz yy yz zz]
     constexpr int P = 64 / APW;
     static_assert(TJ <= 256 && TJ <= NTHR, "queue entries hold 8-bit neighbour indices");
     static_assert(kBondedBlockSpan <= TI, "a bonded block accumulates in the home-tile accumulators");
@@ -376,8 +389,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     // rotated index of a step never wraps
     __shared__ float4 s_js[TJ + 64];
     __shared__ float4 s_je[ANA ? TJ : 1];    // analytic mode: rounding error of each staged fl(x + S)
-    __shared__ unsigned long long s_iacc[3 * TI];
-    __shared__ unsigned long long s_jacc[3 * TJ];
+    __shared__ unsigned long long s_iacc[EN ? 1 : 3 * TI];
+    __shared__ unsigned long long s_jacc[EN ? 1 : 3 * TJ];
     __shared__ unsigned s_q[NWAVE][QCAP];
     __shared__ float2 s_ab[64];
     __shared__ int s_task;
@@ -386,7 +399,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ int s_jo[EXC ? TJ : 1];
     __shared__ int2 s_cinfo[14];             // cells of the task: first slot, atom count
     __shared__ float4 s_cshift[14];          // periodic image shift of each cell
-    __shared__ int s_jslot[2][TJ];           // slot of each neighbour-tile atom (two passes)
+    __shared__ int s_jslot[2][EN ? 1 : TJ];  // slot of each neighbour-tile atom (two passes)
+    __shared__ double s_ered[EN ? NWAVE : 1][8];   // EN: per-wave sums
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -394,15 +408,19 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
 
     // table window and type-pair multipliers
     if (!ANA) {
-        const float *g = a.tab + (size_t)a.win_lo * 6;
         for (int e = tid; e < a.nwin; e += NTHR) {
-            const float *p = g + (size_t)e * 6;
+            const float *p = (EN ? a.etab : a.tab) + (size_t)(a.win_lo + e) * 6;
             float *q = s_tab + e * STR;
             q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[3];
-            if (STR == 6) { q[4] = p[4]; q[5] = p[5]; }
+            if (FSTR == 6) { q[4] = p[4]; q[5] = p[5]; }
         }
     }
-    for (int e = tid; e < a.ntypes * a.ntypes; e += NTHR) s_ab[e] = a.pair_ab[e];
+    // type-pair multipliers: A, B of the force path, or (eps sig^12, eps sig^6) of the energy
+    for (int e = tid; e < a.ntypes * a.ntypes; e += NTHR) s_ab[e] = EN ? a.pair_ab_e[e] : a.pair_ab[e];
+    double eacc[2] = {};   // EN: the task's E and E(rc) on this lane
+    // EN 1: the virial of the task's pairs on this lane, in FP32 (six FP64 sums would push
+    // the scan's registers into scratch); reduced in FP64 from the task's end on
+    float vacc[EN == 1 ? 6 : 1] = {};
 
     float *s_iacc_f = reinterpret_cast<float *>(s_iacc);
     float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
@@ -419,12 +437,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     };
     // bonded tasks are compiled into the exclusion variants only (the others keep their
     // registers); the launcher runs k_bonded beside the others
-    constexpr bool BND = EXC;
+    constexpr bool BND = EXC && !EN;
     const int ntot = a.ntasks + (BND ? a.bd.nblocks : 0);
 
     for (int it = 0;; ++it) {
         int task;
-        if (a.tasks) {
+        if (EN) {
+            // the energy pass: home cells dealt round-robin, one task per cell
+            task = (int)blockIdx.x + it * (int)gridDim.x;
+            if (task >= a.ntasks) break;
+            for (int k = 0; k < 2; ++k) eacc[k] = 0.0;
+            for (int k = 0; k < (EN == 1 ? 6 : 0); ++k) vacc[k] = 0.0f;
+        } else if (a.tasks) {
             // the first task of each workgroup is its own index; later ones come from its
             // queue (only when there are more tasks than workgroups): queue c hands out the
             // tasks c + kQueues * k, k = (workgroups of the queue) + its counter
@@ -517,7 +541,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     s_iom[k] = make_int2(io, (int)a.excl[io]);
                 }
             }
-            for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
+            if (!EN)
+                for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
             if (tid < cur.mj) {
                 float4 sh;
                 const int sl = locate(cur, tid, sh);
@@ -528,10 +553,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 if (tid < 64) s_js[cur.mj + tid] = r;
                 if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
                 s_j[tid] = p;                   // .w: the scaled charge
-                s_jslot[0][tid] = sl;
+                if (!EN) s_jslot[0][tid] = sl;
                 if (EXC) s_jo[tid] = a.orig[sl];
             }
-            for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
+            if (!EN)
+                for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
             lds_barrier();
             int jpar = 0;
 
@@ -587,8 +613,39 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     }
                     const float2 ab = s_ab[e >> 16];
                     const float qq = pa.w * pb.w;
-                    float F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
-                                                               a.off, a.nseg, a.win_lo, a.nwin);
+                    float F = 0.0f, Ee = 0.0f, Ue = 0.0f;
+                    if (EN) {
+                        // energy coefficients of r2's entry from the window (else the global
+                        // table); the virial's force coefficients from the global table, which
+                        // the caches hold (a window of both would halve the entries in LDS)
+                        float c[6] = {0, 0, 0, 0, 0, 0}, ce[6] = {0, 0, 0, 0, 0, 0};
+                        const unsigned wi = (__float_as_uint(r2) >> 15) - (unsigned)(a.off * 256 + a.win_lo);
+                        const size_t idx = (size_t)(lut_seg(r2, a.off, a.nseg) * 256 + lut_bin(r2)) * 6;
+                        if (EN == 1) {
+#pragma unroll
+                            for (int k = 0; k < FSTR; ++k) c[k] = a.tab[idx + k];
+                        }
+                        if (wi < (unsigned)a.nwin) {
+                            const float *t = s_tab + wi * STR;
+#pragma unroll
+                            for (int k = 0; k < FSTR; ++k) ce[k] = t[k];
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < FSTR; ++k) ce[k] = a.etab[idx + k];
+                        }
+                        if (EN == 1) F = fmaf(ab.x, fmaf(c[1], r2, c[0]), -(ab.y * fmaf(c[3], r2, c[2])));
+                        Ee = fmaf(ab.x, fmaf(ce[1], r2, ce[0]), -(ab.y * fmaf(ce[3], r2, ce[2])));
+                        Ue = fmaf(ab.x, a.u12c, -(ab.y * a.u6c));
+                        if (COUL) {
+                            const float q10 = qq * 0.1f;
+                            if (EN == 1) F = fmaf(q10, fmaf(c[5], r2, c[4]), F);
+                            Ee = fmaf(q10, fmaf(ce[5], r2, ce[4]), Ee);
+                            Ue = fmaf(qq, a.u1c, Ue);
+                        }
+                    } else {
+                        F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
+                                                             a.off, a.nseg, a.win_lo, a.nwin);
+                    }
                     // the reference filter 0 < r2 <= rc2 on the pair's own r2 (the scan passed
                     // a superset): one unsigned compare of the bit patterns (r2 >= 0)
                     bool drop = ANA ? (r2 == 0.0f) : (__float_as_uint(r2) - 1u >= __float_as_uint(a.rc2));
@@ -610,7 +667,22 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     }
                     ndrop += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(drop));
                     F = drop ? 0.0f : F;
-                    if (F32) {
+                    if (EN) {
+                        // the pair's energy, its energy at rc and its virial F d (x) d
+                        Ee = drop ? 0.0f : Ee;
+                        Ue = drop ? 0.0f : Ue;
+                        eacc[0] += (double)Ee;
+                        eacc[1] += (double)Ue;
+                        if constexpr (EN == 1) {
+                            const float gx = F * dx, gy = F * dy, gz = F * dz;
+                            vacc[0] += gx * dx;
+                            vacc[1] += gx * dy;
+                            vacc[2] += gx * dz;
+                            vacc[3] += gy * dy;
+                            vacc[4] += gy * dz;
+                            vacc[5] += gz * dz;
+                        }
+                    } else if (F32) {
                         const float fx = F * dx, fy = F * dy, fz = F * dz;
                         atomic_add_wg(&s_iacc_f[3 * ia + 0], fx);
                         atomic_add_wg(&s_iacc_f[3 * ia + 1], fy);
@@ -720,7 +792,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)
                 // and clear the accumulators for the next pass; the atomics stay in flight
                 const int nclr = 3 * max(mj, nxt.mj);
-                if (F32) {
+                if (EN) {
+                    // (no forces in the energy pass)
+                } else if (F32) {
                     for (int k = tid; k < nclr; k += NTHR) {
                         if (k < 3 * mj) {
                             const float v = s_jacc_f[k];
@@ -747,7 +821,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     if (tid < 64) s_js[nxt.mj + tid] = r;
                     if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
                     s_j[tid] = pf;
-                    s_jslot[jpar ^ 1][tid] = sl;
+                    if (!EN) s_jslot[jpar ^ 1][tid] = sl;
                     if (EXC) s_jo[tid] = of;
                 }
                 jpar ^= 1;
@@ -755,7 +829,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 cur = nxt;
             }
             // flush reference-particle forces (D/Partial_Force_Acc.v:65-129)
-            if (F32) {
+            if (EN) {
+                // (no forces in the energy pass)
+            } else if (F32) {
                 for (int k = tid; k < 3 * mi; k += NTHR) {
                     float v = s_iacc_f[k];
                     if (v != 0.0f) atomic_add_agent(&a.f32[4 * (size_t)(hstart + itile + k / 3) + k % 3], v);
@@ -769,7 +845,33 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             }
             lds_barrier();
         }
+        if constexpr (EN) {
+            // the home cell's sums: lanes by a butterfly, then the waves in order; written
+            // to the cell's slots (E, virial 3 x 3, E(rc)), reduced per system in cell order
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) {
+                double v = k < 2 ? eacc[k] : (double)vacc[k < 2 ? 0 : k - 2];
+#pragma unroll
+                for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+                if (lane == 0) s_ered[wave][k] = v;
+            }
+            lds_barrier();
+            if (tid < 8) {
+                double v = 0.0;
+                if (tid < NACC)
+                    for (int w = 0; w < NWAVE; ++w) v += s_ered[w][tid];
+                // E, E(rc), xx xy xz yy yz zz -> slots 0, 10, 1, 2, 3, 5, 6, 9
+                const int slot = tid == 0 ? 0 : tid == 1 ? 10 : tid < 5 ? tid - 1 : tid == 7 ? 9 : tid;
+                double *out = a.cell_red + 11 * (size_t)h;
+                out[slot] = v;
+                if (tid == 3) out[4] = v;    // yx
+                if (tid == 4) out[7] = v;    // zx
+                if (tid == 6) out[8] = v;    // zy
+            }
+            lds_barrier();
+        }
     }
+    if (EN) return;
     // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
         s_stat[wave][0] = npairs - ndrop;
@@ -841,7 +943,7 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
-template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false>
+template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = false, int EN = 0>
 static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st)
 {
     // Two workgroups share a CU's 160 KiB of LDS only while each stays within 80 KiB: the
@@ -852,7 +954,7 @@ static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st
     if (static_lds == ~(size_t)0) {
         hipFuncAttributes fa{};
         static_lds = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(
-                         &k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>)) == hipSuccess
+                         &k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC, EN>)) == hipSuccess
                          ? fa.sharedSizeBytes : 0;
     }
     ForceArgs a = a0;
@@ -866,7 +968,7 @@ static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st
             lds = (size_t)fit * stride;
         }
     }
-    hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC>), grid, dim3(NTHR), lds, st, a);
+    hipLaunchKernelGGL((k_rl_halfshell<COUL, F32, PAR, ANA, APW, TJ, EXC, EN>), grid, dim3(NTHR), lds, st, a);
 }
 
 template <bool COUL, bool EXC>
@@ -1256,149 +1358,141 @@ int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------------
-// Energies.  k_energy: potential energy and virial of each home cell from the c*_{12,6,1}
-// tables (M/FullScale_Simulation_LJArgon.m:938-1010), half shell, written per cell (no
-// atomics).  k_kinetic: full-step velocity v = vh + dv(F) and 0.5 m v^2 per slot.
+// Energies.  The energy pass of k_rl_halfshell (EN): potential energy, virial and the energy
+// at rc of each home cell from the c*_{12,6,1} tables (M/FullScale_Simulation_LJArgon.m:
+// 938-1010), written per cell (no atomics).  k_kinetic: full-step velocity v = vh + dv(F)
+// and 0.5 m v^2 per slot.
 // k_sys_reduce: one workgroup per system sums its cells and slots in a fixed tree order,
 // so the FP64 energies are bit-identical run to run.  Not per step: on request.
 // ---------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_energy(ForceArgs a, const float *etab, const float2 *ab_e,
-                                                double *cell_red)
-{
-    __shared__ double s_r[10][256];
-    const int h = blockIdx.x;
-    const SysGeom &G = a.geom[a.cell_sys[h]];
-    const int nc0 = G.nc[0], nc1 = G.nc[1], nc2 = G.nc[2];
-    const int hl = h - G.cell_off;
-    const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
-    const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
-    double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int ia = threadIdx.x; ia < hcount; ia += blockDim.x) {
-        const float4 pi = a.pos[hstart + ia];
-        const int ti = a.type[hstart + ia];
-        const int oi = a.excl ? a.orig[hstart + ia] : 0;
-        const uint32_t mi = a.excl ? a.excl[oi] : 0u;
-        for (int o = 0; o < 14; ++o) {
-            int rx = hx + c_hs[o][0], ry = hy + c_hs[o][1], rz = hz + c_hs[o][2];
-            float sx = 0.f, sy = 0.f, sz = 0.f;
-            if (rx < 0) { rx += nc0; sx = -G.box[0]; } else if (rx >= nc0) { rx -= nc0; sx = G.box[0]; }
-            if (ry < 0) { ry += nc1; sy = -G.box[1]; } else if (ry >= nc1) { ry -= nc1; sy = G.box[1]; }
-            if (rz < 0) { rz += nc2; sz = -G.box[2]; } else if (rz >= nc2) { rz -= nc2; sz = G.box[2]; }
-            const int c = G.cell_off + (rx * nc1 + ry) * nc2 + rz;
-            const int cs = a.cell_start[c], cc = a.cell_start[c + 1] - cs;
-            for (int jb = (o == 0 ? ia + 1 : 0); jb < cc; ++jb) {
-                float4 pj = a.pos[cs + jb];
-                pj.x = pj.x + sx; pj.y = pj.y + sy; pj.z = pj.z + sz;
-                float dx, dy, dz;
-                float r2 = pair_geom<false>(pi, pj, 0.f, 0.f, 0.f, dx, dy, dz);
-                if (!(r2 > 0.0f && r2 <= a.rc2)) continue;
-                if (a.excl) {
-                    const int d = a.orig[cs + jb] - oi + 16;
-                    if ((unsigned)d < 32u && ((mi >> d) & 1u)) continue;
-                }
-                const int tj = a.type[cs + jb];
-                // ab_e = (eps sig^12, eps sig^6): E = A t12 - B t6, F = A t14 - B t8 (unit
-                // tables 4 r^-12, 4 r^-6, 48 r^-14, 24 r^-8); the scaled charges carry 10x.
-                const float2 abe = ab_e[ti * a.ntypes + tj];
-                const int idx = lut_seg(r2, a.off, a.nseg) * 256 + lut_bin(r2);
-                const float *pe = etab + (size_t)idx * 6;
-                const float *pf = a.tab + (size_t)idx * 6;
-                double t12 = (double)pe[1] * r2 + pe[0], t6 = (double)pe[3] * r2 + pe[2];
-                double E = abe.x * t12 - abe.y * t6;
-                double t14 = (double)pf[1] * r2 + pf[0], t8 = (double)pf[3] * r2 + pf[2];
-                double F = abe.x * t14 - abe.y * t8;
-                if (pi.w != 0.0f && pj.w != 0.0f) {
-                    double qq = (double)pi.w * pj.w;
-                    E += qq * ((double)pe[5] * r2 + pe[4]) * 0.1;
-                    F += qq * ((double)pf[5] * r2 + pf[4]) * 0.1;
-                }
-                e += E;
-                const double d3[3] = {dx, dy, dz};
-                for (int u = 0; u < 3; ++u)
-                    for (int v = 0; v < 3; ++v) w[3 * u + v] += F * d3[u] * d3[v];
-            }
-        }
-    }
-    s_r[0][threadIdx.x] = e;
-    for (int k = 0; k < 9; ++k) s_r[k + 1][threadIdx.x] = w[k];
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int k = 0; k < 10; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x < 10) cell_red[10 * (size_t)h + threadIdx.x] = s_r[threadIdx.x][0];
-}
-
-__global__ __launch_bounds__(256) void k_kinetic(int n, const float4 *vel, const int *type,
+// One workgroup per kinetic-energy chunk (at most kKeChunk consecutive slots of one system):
+// full-step velocities v = vh + dv(F) per slot (written to vout when given), 0.5 m v^2
+// summed by each thread over its slots in order and then by a fixed tree.
+__global__ __launch_bounds__(256) void k_kinetic(const int2 *chunks, const float4 *vel, const int *type,
                                                  const long long *fix, const float *f32,
                                                  const double *hk, const double *mass,
-                                                 int first, float4 *vout, double *atom_ke)
+                                                 int first, float4 *vout, double *chunk_ke)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int t = type[s];
-    const double k = hk[t];
-    float4 v = vel[s];
-    float vi[3] = {v.x, v.y, v.z};
-    double vv = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        float dv = f32 ? (float)((double)f32[4 * (size_t)s + d] * 4294967296.0 * k)
-                       : (float)((double)fix[3 * (size_t)s + d] * k);
-        float vf = first ? vi[d] : vi[d] + dv;
-        vi[d] = vf;
-        vv += (double)vf * (double)vf;
+    __shared__ double s_r[256];
+    const int2 ch = chunks[blockIdx.x];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < ch.y; k += 256) {
+        const int s = ch.x + k;
+        const int t = type[s];
+        const double hkt = hk[t];
+        const float4 v = vel[s];
+        float vi[3] = {v.x, v.y, v.z};
+        double vv = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            float dv = f32 ? (float)((double)f32[4 * (size_t)s + d] * 4294967296.0 * hkt)
+                           : (float)((double)fix[3 * (size_t)s + d] * hkt);
+            float vf = first ? vi[d] : vi[d] + dv;
+            vi[d] = vf;
+            vv += (double)vf * (double)vf;
+        }
+        const double ke = 0.5 * mass[t] * vv * 1.0e4;
+        acc += ke;
+        if (vout) vout[s] = make_float4(vi[0], vi[1], vi[2], (float)ke);
     }
-    const double ke = 0.5 * mass[t] * vv * 1.0e4;
-    atom_ke[s] = ke;
-    if (vout) vout[s] = make_float4(vi[0], vi[1], vi[2], (float)ke);
-}
-
-__global__ __launch_bounds__(256) void k_sys_reduce(const SysGeom *geom, const double *cell_red,
-                                                    const double *atom_ke, const double *term_e,
-                                                    const int *term_off, double *red)
-{
-    __shared__ double s_r[12][256];
-    const SysGeom &G = geom[blockIdx.x];
-    const int ncell = G.nc[0] * G.nc[1] * G.nc[2];
-    double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < G.n; k += blockDim.x) acc[0] += atom_ke[G.atom_off + k];
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x)
-        for (int k = 0; k < 10; ++k) acc[1 + k] += cell_red[10 * (size_t)(G.cell_off + c) + k];
-    if (term_e)
-        for (int t = term_off[blockIdx.x] + threadIdx.x; t < term_off[blockIdx.x + 1]; t += blockDim.x)
-            acc[11] += term_e[t];
-    for (int k = 0; k < 12; ++k) s_r[k][threadIdx.x] = acc[k];
+    s_r[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int k = 0; k < 12; ++k) s_r[k][threadIdx.x] += s_r[k][threadIdx.x + off];
+        if ((int)threadIdx.x < off) s_r[threadIdx.x] += s_r[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x < 12) red[kRedStride * (size_t)blockIdx.x + threadIdx.x] = s_r[threadIdx.x][0];
+    if (threadIdx.x == 0) chunk_ke[blockIdx.x] = s_r[0];
 }
 
-int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st)
+// One workgroup per system: its kinetic-energy chunks, its cells' sums (energy, virial,
+// energy at rc) and its bonded terms' energies, each thread over a fixed stride and then a
+// fixed tree, so the FP64 energies are bit-identical run to run.
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void k_sys_reduce(const SysGeom *geom, const double *cell_red,
+                                                            const double *chunk_ke, const double *term_e,
+                                                            const int *term_off, double *red)
+{
+    __shared__ double s_r[kRedStride][kRedThreads / 64];
+    const SysGeom &G = geom[blockIdx.x];
+    const int ncell = G.nc[0] * G.nc[1] * G.nc[2];
+    double acc[kRedStride] = {};
+    for (int k = threadIdx.x; k < G.ke_n; k += kRedThreads) acc[0] += chunk_ke[G.ke_off + k];
+    for (int c = threadIdx.x; c < ncell; c += kRedThreads) {
+        for (int k = 0; k < 10; ++k) acc[1 + k] += cell_red[11 * (size_t)(G.cell_off + c) + k];
+        acc[12] += cell_red[11 * (size_t)(G.cell_off + c) + 10];
+    }
+    if (term_e)
+        for (int t = term_off[blockIdx.x] + threadIdx.x; t < term_off[blockIdx.x + 1]; t += kRedThreads)
+            acc[11] += term_e[t];
+    // within each wave by a butterfly, then over the waves in order
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < kRedStride; ++k) {
+        double v = acc[k];
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        if (lane == 0) s_r[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRedStride) {
+        double v = 0.0;
+        for (int q = 0; q < kRedThreads / 64; ++q) v += s_r[threadIdx.x][q];
+        red[kRedStride * (size_t)blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st, int mode)
 {
     if (s.n > 0) {
         const bool f32 = s.flags & MDRL_ACC_FP32;
-        hipLaunchKernelGGL(k_kinetic, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.vel, s.type,
+        hipLaunchKernelGGL(k_kinetic, dim3(s.nke), dim3(256), 0, st, s.ke_chunks, s.vel, s.type,
                            f32 ? nullptr : s.fix, f32 ? s.f32 : nullptr, s.hk, s.mass,
-                           s.first_step ? 1 : 0, vout, s.atom_ke);
+                           s.first_step ? 1 : 0, vout, s.chunk_ke);
+        if (mode == kEnergyKinetic) {   // full-step velocities only
+            MDRL_HIP(hipGetLastError());
+            return MDRL_OK;
+        }
+        // the energy pass: every home cell with all 14 offsets, the default tile shape, cells
+        // dealt round-robin to a fixed grid (so the sums do not depend on the force schedule)
         ForceArgs a{};
         a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
         a.geom = s.geom; a.cell_sys = s.cell_sys;
-        a.rc2 = s.rc2;
-        a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
-        a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
+        a.tasks = nullptr;
+        a.ntasks = s.ncells;
+        a.rc2 = s.rc2; a.rc2_scan = s.rc2_scan;
+        a.tab = s.tab; a.etab = s.etab; a.off = s.exp_offset; a.nseg = s.nseg;
+        a.win_lo = s.win.lo; a.nwin = s.win.n;
+        a.pair_ab = s.pair_ab; a.pair_ab_e = s.pair_ab_e; a.ntypes = s.ntypes;
+        a.u12c = s.u12c; a.u6c = s.u6c; a.u1c = s.u1c;
+        a.cell_red = s.cell_red;
+        a.counters = s.counters;
+        a.ngroups = 1;
+        offset_groups(1, a.gstart);
         a.orig = s.orig; a.excl = s.excl;
-        hipLaunchKernelGGL(k_energy, dim3(s.ncells), dim3(256), 0, st, a, s.etab, s.pair_ab_e,
-                           s.cell_red);
+        const int grid_n = std::min(s.ncells, 2 * s.cu_count);
+        a.progress_prio = s.ncells <= 2 * grid_n;
+        const bool vir = mode == kEnergyVirial;
+        const size_t lds = (size_t)s.win.n * (s.coul ? 6 : 4) * sizeof(float);
+        const dim3 g(grid_n);
+        if (vir) {
+            if (s.coul) {
+                if (s.excl) launch_rl<true, false, false, false, 64, 256, true, 1>(a, g, lds, st);
+                else launch_rl<true, false, false, false, 64, 256, false, 1>(a, g, lds, st);
+            } else {
+                if (s.excl) launch_rl<false, false, false, false, 64, 256, true, 1>(a, g, lds, st);
+                else launch_rl<false, false, false, false, 64, 256, false, 1>(a, g, lds, st);
+            }
+        } else {
+            if (s.coul) {
+                if (s.excl) launch_rl<true, false, false, false, 64, 256, true, 2>(a, g, lds, st);
+                else launch_rl<true, false, false, false, 64, 256, false, 2>(a, g, lds, st);
+            } else {
+                if (s.excl) launch_rl<false, false, false, false, 64, 256, true, 2>(a, g, lds, st);
+                else launch_rl<false, false, false, false, 64, 256, false, 2>(a, g, lds, st);
+            }
+        }
         if (s.nterms > 0)
             hipLaunchKernelGGL(k_bonded_energy, dim3(nblk(s.nterms, 256)), dim3(256), 0, st, bonded_args(s));
     }
-    hipLaunchKernelGGL(k_sys_reduce, dim3(s.nsys), dim3(256), 0, st, s.geom, s.cell_red, s.atom_ke,
+    hipLaunchKernelGGL(k_sys_reduce, dim3(s.nsys), dim3(kRedThreads), 0, st, s.geom, s.cell_red, s.chunk_ke,
                        s.nterms > 0 ? s.term_e : nullptr, s.term_off, s.red);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;

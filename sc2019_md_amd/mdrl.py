@@ -83,6 +83,13 @@ class CModelCfg(ctypes.Structure):
     ]
 
 
+class EnergyTerms(ctypes.Structure):
+    _fields_ = [("kinetic", ctypes.c_double), ("potential", ctypes.c_double),
+                ("range_limited", ctypes.c_double), ("bonded", ctypes.c_double),
+                ("long_range", ctypes.c_double), ("virial", ctypes.c_double * 9),
+                ("cutoff_shift", ctypes.c_double)]
+
+
 class SystemDesc(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int), ("ntypes", ctypes.c_int), ("box", ctypes.c_float * 3),
@@ -165,6 +172,7 @@ def lib():
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]
         L.md_rl_set_graph.argtypes = [P, I]
+        L.md_rl_get_energy_terms.argtypes = [P, I, ctypes.POINTER(EnergyTerms)]
         L.md_rl_graph_prepare.argtypes = [P, I]
         L.md_rl_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
         L.md_rl_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
@@ -452,11 +460,23 @@ class System:
             self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), self.n))
         return out
 
-    def energy(self):
+    def energy(self, virial=True):
+        """md_rl_energy: (potential, kinetic, virial 3x3 or None); virial=False skips the
+        pair virial (a cheaper energy pass)."""
         pe, ke = ctypes.c_double(), ctypes.c_double()
-        vir = (ctypes.c_double * 9)()
+        vir = (ctypes.c_double * 9)() if virial else None
         _check(lib().md_rl_energy(self.handle, ctypes.byref(pe), ctypes.byref(ke), vir))
-        return pe.value, ke.value, np.array(vir[:]).reshape(3, 3)
+        return pe.value, ke.value, (np.array(vir[:]).reshape(3, 3) if virial else None)
+
+    def energy_terms(self, sys=-1):
+        """md_rl_get_energy_terms: kinetic, potential and its parts, virial, and the sum of
+        the in-cutoff pairs' energies at rc (potential - cutoff_shift is the shifted
+        potential that velocity Verlet conserves with the kinetic energy)."""
+        t = EnergyTerms()
+        _check(lib().md_rl_get_energy_terms(self.handle, sys, ctypes.byref(t)))
+        return {"kinetic": t.kinetic, "potential": t.potential, "range_limited": t.range_limited,
+                "bonded": t.bonded, "long_range": t.long_range,
+                "virial": np.array(t.virial[:]).reshape(3, 3), "cutoff_shift": t.cutoff_shift}
 
     def stats(self):
         s = Stats()

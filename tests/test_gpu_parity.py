@@ -331,6 +331,28 @@ def test_virial_vs_fp64(which, index):
     assert np.trace(vir) == pytest.approx(np.trace(vir64), rel=2e-4)
 
 
+def test_energy_terms_consistent():
+    """md_rl_get_energy_terms against md_rl_energy on the same state (bit-identical kinetic,
+    potential and virial), the virial-free pass's potential bit-identical to the full pass's,
+    and the cutoff shift: one U(rc) = 4 eps ((sigma/rc)^12 - (sigma/rc)^6) per in-cutoff
+    pair (argon, one species), within the tables' 1e-4."""
+    s = mdrl.SyntheticSystem(1)
+    g = gpu_system(s)
+    g.step(10)
+    pe, ke, vir = g.energy()
+    pe2, ke2, none = g.energy(virial=False)
+    t = g.energy_terms()
+    npairs = g.stats().pair_evals
+    g.close()
+    assert none is None and pe2 == pe and ke2 == ke
+    assert t["potential"] == pe and t["kinetic"] == ke and np.array_equal(t["virial"], vir)
+    assert t["range_limited"] == pe and t["bonded"] == 0.0 and t["long_range"] == 0.0
+    u_rc = 4 * 0.996 * ((3.4 / 8.5) ** 12 - (3.4 / 8.5) ** 6)
+    say("cutoff shift %.6f over %d pairs: %.6e per pair (U(rc) %.6e)"
+        % (t["cutoff_shift"], npairs, t["cutoff_shift"] / npairs, u_rc))
+    assert t["cutoff_shift"] / npairs == pytest.approx(u_rc, rel=1e-4)
+
+
 def test_nve_energy_drift():
     """Argon, 2,000 steps of 2 fs.  The reference potential is truncated without a shift
     (the *_no_smooth tables), so the potential jumps by U(rc) whenever a pair crosses the
