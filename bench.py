@@ -336,8 +336,8 @@ def batched_vs_plain(nrep, warmup, steps, seed, table):
 
 def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     """The precision configuration: int64 forces bit-identical over 10 repeat
-    evaluations, FP32 atomics vs int64, FP64-reduced total energy along NVE runs (one per
-    time step in dt_list).  Returns the leg's record and, for the CPU leg's comparison with
+    evaluations, FP32 atomics vs int64, FP64-reduced total and shifted energies along NVE
+    runs (one per time step in dt_list), sampled ten times per run.  Returns the leg's record and, for the CPU leg's comparison with
     the FP64 direct oracle, the relaxed positions and both force sets."""
     import hashlib
 
@@ -371,8 +371,16 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
         g = mdrl.System.from_synthetic(sd)
         g.set_particles(relaxed)
         g.set_graph(True)
-        pe0, ke0, _ = g.energy()
-        trace = [(0, pe0 + ke0)]
+
+        def sample(step):
+            # raw KE + PE, and the shifted energy KE + PE - cutoff_shift (one U(rc) per
+            # in-cutoff pair): the quantity velocity Verlet conserves on a potential that is
+            # truncated without a shift
+            t = g.energy_terms()
+            return (step, t["kinetic"] + t["potential"], t["kinetic"] + t["potential"] - t["cutoff_shift"],
+                    t["kinetic"])
+
+        trace = [sample(0)]
         every = max(1, nve_steps // 10)
         t0 = time.perf_counter()
         done = 0
@@ -380,16 +388,18 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
             k = min(every, nve_steps - done)
             g.step_async(k)
             done += k
-            pe, ke, _ = g.energy()
-            trace.append((done, pe + ke))
+            trace.append(sample(done))
         wall = time.perf_counter() - t0
         g.close()
-        e0 = trace[0][1]
-        out["nve"].append({"dt_fs": dt, "steps": nve_steps, "E0_kj_mol": e0,
-                           "E_end_kj_mol": trace[-1][1],
+        _, e0, h0, k0 = trace[0]
+        out["nve"].append({"dt_fs": dt, "steps": nve_steps, "KE0_kj_mol": k0,
+                           "E0_kj_mol": e0, "E_end_kj_mol": trace[-1][1],
+                           "shifted_E0_kj_mol": h0, "shifted_E_end_kj_mol": trace[-1][2],
                            "rel_drift": (trace[-1][1] - e0) / abs(e0),
-                           "max_rel_dev": max(abs(e - e0) for _, e in trace) / abs(e0),
-                           "trace": trace, "wall_s": wall})
+                           "max_rel_dev": max(abs(e - e0) for _, e, _, _ in trace) / abs(e0),
+                           "shifted_drift_vs_KE0": (trace[-1][2] - h0) / k0,
+                           "shifted_max_dev_vs_KE0": max(abs(h - h0) for _, _, h, _ in trace) / k0,
+                           "trace_step_raw_shifted_kinetic": trace, "wall_s": wall})
     return out, relaxed, {"int64": fix, "fp32": f32a}
 
 
