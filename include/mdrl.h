@@ -21,6 +21,7 @@
 #ifndef MDRL_H
 #define MDRL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

@@ -132,6 +132,10 @@ extern "C" int md_rl_table_from_arrays(md_rl_table **out, int exp_offset, int ns
 extern "C" int md_rl_table_load(md_rl_table **out, const char *dir, int exp_offset, int nseg)
 {
     if (!out || !dir) { set_error("md_rl_table_load: bad argument"); return MDRL_EINVAL; }
+    if (nseg <= 0 || nseg > 32 || exp_offset <= 0 || exp_offset + nseg > 255) {
+        set_error("md_rl_table_load: bad geometry (offset %d, nseg %d)", exp_offset, nseg);
+        return MDRL_EINVAL;
+    }
     const int nent = nseg * 256;
     std::vector<float> buf0[MDRL_NTERMS], buf1[MDRL_NTERMS];
     const float *p0[MDRL_NTERMS] = {}, *p1[MDRL_NTERMS] = {};
