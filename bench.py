@@ -187,7 +187,7 @@ def analytic_leg(s, relaxed, fd_relaxed, warmup, steps, cpu_value):
     g = mdrl.System.from_synthetic(s, flags=mdrl.EVAL_ANALYTIC)
     g.set_particles(relaxed)
     err_relaxed = force_error(g.forces_fixed(), fd_relaxed)
-    sched, _ = g.autotune(reps=2)
+    sched, _, _ = g.autotune(reps=2, cache=True)
     step_s = run_timed(g, warmup, steps, True)
     pairs = g.stats().pair_evals
     fix_end, pos_end = g.forces_fixed(), g.particles()
@@ -262,7 +262,7 @@ def plain_contexts(systems, table, relax=0, relax_step=0.02, tune=False):
         if relax:
             g.minimize(relax, relax_step, 1e-4)
         if tune:
-            g.autotune(reps=1)
+            g.autotune(reps=1, cache=True)
     return plain
 
 
@@ -298,7 +298,7 @@ def batched_vs_plain(nrep, warmup, steps, seed, table):
     systems = [mdrl.SyntheticSystem(4, k, seed=seed) for k in range(nrep)]
     natoms = sum(s.n for s in systems)
     b = mdrl.Batch.from_synthetic(systems, table=table)
-    sched, tune_ms = b.autotune(reps=2)
+    sched, tune_ms, tune_hit = b.autotune(reps=2, cache=True)
     tb = run_timed(b, warmup, steps, True)
     pairs_b = b.stats().pair_evals
     dig_b = state_digest([(b, k) for k in range(nrep)])
@@ -309,7 +309,7 @@ def batched_vs_plain(nrep, warmup, steps, seed, table):
         "steps": steps,
         "warmup": warmup,
         "batched": {"ms_per_step": tb * 1e3, "pair_evals_per_s": pairs_b / tb,
-                    "schedule": sched.as_dict(), "graph": True},
+                    "schedule": dict(sched.as_dict(), autotune_cache_hit=tune_hit), "graph": True},
         "pair_evals_per_step": pairs_b,
     }
     digests = {"batched": dig_b}
@@ -417,7 +417,7 @@ def bench_batch(args, which):
     relax = args.relax if args.relax else 0
     if relax:
         b.minimize(relax, args.relax_step, 1e-4)
-    sched, tune_ms = b.autotune(reps=3)
+    sched, tune_ms, tune_hit = b.autotune(reps=3, cache=True)
     b.step(args.warmup)
     pe0, ke0, _ = b.energy()
     b.set_graph(bool(args.graph))
@@ -498,7 +498,7 @@ def bench_batch(args, which):
         "ns_per_day": (1e3 / ms) * s0.dt * 0.0864,
         "force_kernel_ms": st.force_ms,
         "step_device_ms_eager": st.step_ms,
-        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms),
+        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms, autotune_cache_hit=tune_hit),
         "graph_captured_steps": bool(args.graph),
         "energy_kj_mol": {"potential_start": pe0, "kinetic_start": ke0, "potential_end": pe1,
                           "kinetic_end": ke1,
@@ -562,7 +562,7 @@ def main():
     relaxed = g.particles()
     fix_relaxed = g.forces_fixed()     # the forces the timed steps start from
 
-    sched, tune_ms = g.autotune(reps=3)
+    sched, tune_ms, tune_hit = g.autotune(reps=3, cache=True)
     field_buf = None
     if args.long_range_field:
         # F_i = q_i E with E = (0.8, -0.3, 0.45) kJ/mol/A/e, input order
@@ -631,7 +631,7 @@ def main():
         },
         "ns_per_day": ns_day,
         "force_kernel_ms": st.force_ms,
-        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms),
+        "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms, autotune_cache_hit=tune_hit),
         "dt_fs": s.dt,
         "step_device_ms_eager": st.step_ms,
         "graph_captured_steps": bool(args.graph),

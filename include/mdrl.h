@@ -338,6 +338,15 @@ int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *s);
 /* Times every available schedule on the current positions (reps force evaluations
  * each, HIP events) and keeps the fastest. */
 int md_rl_autotune(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms);
+/* md_rl_autotune through a cache of winning schedules keyed by device, library build and
+ * workload shape (atoms, systems, cells, types, rc, flags, Coulomb, exclusions, bonded
+ * terms): a hit applies the cached schedule (and evaluates forces, as a sweep leaves them)
+ * without sweeping; a miss sweeps and records.  The cache lives in the process and in the
+ * JSON Lines file $MDRL_TUNE_CACHE ("" or "off": none; default
+ * $XDG_CACHE_HOME/mdrl/schedules.jsonl or ~/.cache/mdrl/schedules.jsonl).  hit (may be
+ * NULL) receives 1 on a cache hit, 0 after a sweep.  No reference counterpart (the
+ * reference's FPGA schedule is fixed at synthesis). */
+int md_rl_autotune_cached(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms, int *hit);
 
 /* ---- the C model entry point ------------------------------------------------------- */
 /* Drop-in for int MD_Evaluation_Model() (R/C_Model_ForceEval/MD_Evaluation_Model.h:1):

@@ -1,11 +1,16 @@
 // mdrl_api.cpp -- the C ABI: pair batches, systems (contexts) and the C-model entry.
+#include <fcntl.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1349,6 +1354,138 @@ out:
     if ((rc = md_rl_force_eval(c))) return rc;
     if (best) *best = win;
     if (best_ms) *best_ms = win_ms;
+    return MDRL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// autotuner cache: the winning schedule per device, library build and workload shape,
+// kept in the process and in a JSON Lines file, one object per line:
+//   {"key": "...", "atoms_per_wave": 64, "tile_depth": 256, "offset_groups": 2,
+//    "persistent": 1, "ms": 0.2134}
+// File: $MDRL_TUNE_CACHE ("" or "off": none), else $XDG_CACHE_HOME/mdrl/schedules.jsonl,
+// else $HOME/.cache/mdrl/schedules.jsonl.  Lines that do not parse are skipped; new
+// results are appended (one write per line).
+// ---------------------------------------------------------------------------------
+
+namespace {
+
+struct TuneEntry {
+    md_rl_schedule s;
+    double ms;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneEntry> g_tune;
+
+// what a schedule's speed depends on; a rebuilt library (other kernels) gets new keys
+std::string tune_key(const md_rl_ctx *c)
+{
+    const DeviceSystem &s = c->s;
+    char arch[64] = "unknown";
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        snprintf(arch, sizeof arch, "%s", prop.gcnArchName);
+        for (char *q = arch; *q; ++q)
+            if (*q == '"' || *q == '\\' || *q == ' ') *q = '_';
+    }
+    uint32_t rc2;
+    memcpy(&rc2, &s.rc2, 4);
+    char k[512];
+    snprintf(k, sizeof k,
+             "%s cu%d build%d-%s-%s atoms%d systems%d cells%d types%d rc2x%08x flags%d coul%d excl%d terms%d",
+             arch, s.cu_count, MDRL_VERSION, __DATE__, __TIME__, s.n, s.nsys, s.ncells, s.ntypes, rc2, s.flags,
+             s.coul ? 1 : 0, s.excl ? 1 : 0, s.nterms);
+    for (char *q = k; *q; ++q)
+        if (*q == '"' || *q == '\\') *q = '_';
+    return k;
+}
+
+std::string tune_file()
+{
+    const char *env = getenv("MDRL_TUNE_CACHE");
+    if (env) return (*env == 0 || strcmp(env, "off") == 0) ? std::string() : std::string(env);
+    const char *xdg = getenv("XDG_CACHE_HOME");
+    if (xdg && *xdg) return std::string(xdg) + "/mdrl/schedules.jsonl";
+    const char *home = getenv("HOME");
+    if (home && *home) return std::string(home) + "/.cache/mdrl/schedules.jsonl";
+    return std::string();
+}
+
+// the last entry of the file for key (later lines win)
+bool tune_file_lookup(const std::string &path, const std::string &key, TuneEntry &out)
+{
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[1024], k[600];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+        TuneEntry e{};
+        int used = 0;
+        if (sscanf(line, " {\"key\": \"%599[^\"]\", \"atoms_per_wave\": %d, \"tile_depth\": %d, "
+                         "\"offset_groups\": %d, \"persistent\": %d, \"ms\": %lf}%n",
+                   k, &e.s.atoms_per_wave, &e.s.tile_depth, &e.s.offset_groups, &e.s.persistent, &e.ms, &used) == 6 &&
+            used > 0 && key == k) {
+            out = e;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+void tune_file_append(const std::string &path, const std::string &key, const TuneEntry &e)
+{
+    // create the directories of the default location
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
+    char line[1024];
+    const int len = snprintf(line, sizeof line,
+                             "{\"key\": \"%s\", \"atoms_per_wave\": %d, \"tile_depth\": %d, \"offset_groups\": %d, "
+                             "\"persistent\": %d, \"ms\": %.6f}\n",
+                             key.c_str(), e.s.atoms_per_wave, e.s.tile_depth, e.s.offset_groups, e.s.persistent, e.ms);
+    if (len <= 0 || len >= (int)sizeof line) return;
+    const int fd = open(path.c_str(), O_WRONLY | O_APPEND | O_CREAT, 0644);
+    if (fd < 0) return;                     // an unwritable cache only costs a re-tune
+    if (write(fd, line, (size_t)len) != len) { /* a short write leaves a line the reader skips */ }
+    close(fd);
+}
+
+}  // namespace
+
+extern "C" int md_rl_autotune_cached(md_rl_ctx *c, int reps, md_rl_schedule *best, double *best_ms, int *hit)
+{
+    if (!c || reps < 1) { set_error("md_rl_autotune_cached: bad argument"); return MDRL_EINVAL; }
+    int rc = ensure_uploaded(c);
+    if (rc) return rc;
+    const std::string key = tune_key(c), path = tune_file();
+    TuneEntry e{};
+    bool found;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(key);
+        found = it != g_tune.end();
+        if (found) e = it->second;
+        else if (!path.empty() && tune_file_lookup(path, key, e)) {
+            found = true;
+            g_tune[key] = e;
+        }
+    }
+    if (found && md_rl_set_schedule(c, &e.s) == MDRL_OK) {
+        if ((rc = md_rl_force_eval(c))) return rc;   // forces for the current positions, as a sweep leaves them
+        if (best) *best = e.s;
+        if (best_ms) *best_ms = e.ms;
+        if (hit) *hit = 1;
+        return MDRL_OK;
+    }
+    if ((rc = md_rl_autotune(c, reps, &e.s, &e.ms))) return rc;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune[key] = e;
+        if (!path.empty()) tune_file_append(path, key, e);
+    }
+    if (best) *best = e.s;
+    if (best_ms) *best_ms = e.ms;
+    if (hit) *hit = 0;
     return MDRL_OK;
 }
 

@@ -177,6 +177,8 @@ def lib():
         L.md_rl_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
         L.md_rl_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
         L.md_rl_autotune.argtypes = [P, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_double)]
+        L.md_rl_autotune_cached.argtypes = [P, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(I)]
         L.md_rl_cmodel_default.argtypes = [ctypes.POINTER(CModelCfg)]
         L.md_rl_cmodel_default.restype = None
         L.md_rl_cmodel_run.argtypes = [ctypes.POINTER(CModelCfg), F, F]
@@ -502,9 +504,16 @@ class System:
         _check(lib().md_rl_get_schedule(self.handle, ctypes.byref(sc)))
         return sc
 
-    def autotune(self, reps=3):
+    def autotune(self, reps=3, cache=False):
+        """Sweeps the schedules (md_rl_autotune) and keeps the fastest: (schedule, ms).
+        cache=True goes through md_rl_autotune_cached and returns (schedule, ms, hit)."""
         sc = Schedule()
         ms = ctypes.c_double()
+        if cache:
+            hit = ctypes.c_int()
+            _check(lib().md_rl_autotune_cached(self.handle, reps, ctypes.byref(sc), ctypes.byref(ms),
+                                               ctypes.byref(hit)))
+            return sc, ms.value, bool(hit.value)
         _check(lib().md_rl_autotune(self.handle, reps, ctypes.byref(sc), ctypes.byref(ms)))
         return sc, ms.value
 

@@ -91,6 +91,9 @@ def test_null_context_rejected_with_a_message():
         "md_rl_set_schedule": lambda: L.md_rl_set_schedule(None, None),
         "md_rl_get_schedule": lambda: L.md_rl_get_schedule(None, None),
         "md_rl_autotune": lambda: L.md_rl_autotune(None, 1, None, None),
+        "md_rl_autotune_cached": lambda: L.md_rl_autotune_cached(None, 1, None, None, None),
+        "md_rl_get_energy_terms": lambda: L.md_rl_get_energy_terms(None, -1, None),
+        "md_rl_graph_prepare": lambda: L.md_rl_graph_prepare(None, 2),
         "md_rl_get_stats": lambda: L.md_rl_get_stats(None, None),
         "md_rl_energy": lambda: L.md_rl_energy(None, None, None, None),
         "md_rl_step_async": lambda: L.md_rl_step_async(None, 1, None),

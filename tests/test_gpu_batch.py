@@ -111,6 +111,59 @@ def test_autotune_picks_a_schedule():
     g.close()
 
 
+CACHE_CHILD = """
+import sys
+sys.path.insert(0, %r)
+from sc2019_md_amd import mdrl
+g = mdrl.System.from_synthetic(mdrl.SyntheticSystem(1))
+sc, ms, hit = g.autotune(reps=1, cache=True)
+print(int(hit), sc.atoms_per_wave, sc.tile_depth, sc.offset_groups, sc.persistent, flush=True)
+g.close()
+"""
+
+
+def test_autotune_cache(tmp_path, monkeypatch):
+    """md_rl_autotune_cached: a miss sweeps and appends one JSON line to $MDRL_TUNE_CACHE
+    (lines that do not parse are skipped); a second context of the same shape hits with the
+    same schedule and bit-identical forces; a fresh process hits from the file alone; a
+    batch of another shape misses and appends its own line."""
+    import json
+    import os
+    import subprocess
+    import sys
+    path = tmp_path / "schedules.jsonl"
+    path.write_text("not json\n{\"key\": \"truncated\n")
+    monkeypatch.setenv("MDRL_TUNE_CACHE", str(path))
+    s = mdrl.SyntheticSystem(1)
+    g = mdrl.System.from_synthetic(s)
+    sc, ms, hit = g.autotune(reps=1, cache=True)
+    f0 = g.forces_fixed()
+    g.close()
+    say("autotune cache: miss ->", sc.as_dict(), "%.4f ms" % ms)
+    assert not hit and ms > 0
+    rows = [json.loads(l) for l in path.read_text().splitlines()[2:]]
+    assert len(rows) == 1 and {k: rows[0][k] for k in sc.as_dict()} == sc.as_dict()
+    assert "gfx950" in rows[0]["key"] and "atoms512 " in rows[0]["key"]
+
+    h = mdrl.System.from_synthetic(s)
+    sc2, ms2, hit2 = h.autotune(reps=1, cache=True)
+    assert hit2 and sc2.as_dict() == sc.as_dict() and ms2 == ms
+    assert np.array_equal(h.forces_fixed(), f0)
+    h.close()
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", CACHE_CHILD % root], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, MDRL_TUNE_CACHE=str(path)))
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = [int(v) for v in r.stdout.split()]
+    assert got == [1, sc.atoms_per_wave, sc.tile_depth, sc.offset_groups, sc.persistent]
+
+    b = mdrl.Batch.from_synthetic(replicas(IDX))
+    _, _, hit3 = b.autotune(reps=1, cache=True)
+    b.close()
+    assert not hit3 and len(path.read_text().splitlines()) == 4
+
+
 @pytest.mark.parametrize("which", ["batch", "single"])
 def test_graph_steps_bit_identical(which):
     """hipGraph-captured steps (replayed many times, with eager work between replays)
