@@ -312,6 +312,8 @@ __device__ __forceinline__ void bonded_block(const BondedArgs &a, int4 blk, unsi
 // 12 waves per workgroup: two workgroups (24 waves) fill a CU's LDS and its VGPRs at
 // 6 waves per SIMD.
 constexpr int TI = 256, QCAP = 128, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
+// queue entry: home index (8 bits) | neighbour index (9 bits) << 8 | type pair << 17
+constexpr unsigned kJbShift = 8, kJbMask = 0x1ffu, kTpShift = 17;
 static_assert(TI <= 256, "queue entries hold 8-bit home indices");
 
 // One neighbour tile: offsets [o, oe) of the task's group, each a whole cell (cells that
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     constexpr int STR = FSTR;            // floats per window entry
     constexpr int NACC = EN == 1 ? 8 : 2;   // EN: E, E(rc)[, W xx xy xz yy yz zz]
     constexpr int P = 64 / APW;
-    static_assert(TJ <= 256 && TJ <= NTHR, "queue entries hold 8-bit neighbour indices");
+    static_assert(TJ <= 512 && TJ <= NTHR, "queue entries hold 9-bit neighbour indices");
     static_assert(kBondedBlockSpan <= TI, "a bonded block accumulates in the home-tile accumulators");
     extern __shared__ float s_tab[];
     __shared__ float4 s_i[TI];
@@ -548,7 +550,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const int sl = locate(cur, tid, sh);
                 float4 p = a.pos[sl];
                 if (ANA) s_je[tid] = shift_error(p, sh);
-                const float4 r = scan_rec(p, sh, ((unsigned)tid << 8) + ((unsigned)a.type[sl] << 16));
+                const float4 r = scan_rec(p, sh, ((unsigned)tid << kJbShift) + ((unsigned)a.type[sl] << kTpShift));
                 s_js[tid] = r;
                 if (tid < 64) s_js[cur.mj + tid] = r;
                 if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
@@ -591,11 +593,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const float4 csh = s_cshift[cur.o];   // the tile's shift (single-cell tiles)
                 const float sx = csh.x, sy = csh.y, sz = csh.z;
 
-                // queue entry: home index + neighbour index << 8 + type pair << 16.  The scan
+                // queue entry: home index + neighbour index << 8 + type pair << 17.  The scan
                 // queues r2 <= rc2; a pair at r2 == 0 (coincident atoms, outside the
                 // reference's 0 < r2 filter) or an excluded pair is dropped here, uncounted.
                 auto eval = [&](unsigned e) {
-                    const int ia = (int)(e & 0xffu), jb = (int)((e >> 8) & 0xffu);
+                    const int ia = (int)(e & 0xffu), jb = (int)((e >> kJbShift) & kJbMask);
                     const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
                     float4 je;
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     } else {
                         r2 = pair_geom<PAR>(pa, pb, sx, sy, sz, dx, dy, dz);
                     }
-                    const float2 ab = s_ab[e >> 16];
+                    const float2 ab = s_ab[e >> kTpShift];
                     const float qq = pa.w * pb.w;
                     float F = 0.0f, Ee = 0.0f, Ue = 0.0f;
                     if (EN) {
@@ -737,7 +739,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         } else {
                             hx2[ch] = 0.0f; hy2[ch] = 0.0f; hz2[ch] = 0.0f; hc[ch] = 1e30f;
                         }
-                        eb[ch] = (unsigned)ia + ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << 16);
+                        eb[ch] = (unsigned)ia + ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << kTpShift);
                     }
                     // SELF: the home cell, where a pair needs jb > itile + ia - jtile
                     // (compiled twice so the other 13 offsets skip that test)
@@ -749,7 +751,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                             const bool kv = (P == 1) || (k < mj);
                             const float4 pj = s_js[kv ? bj + k : 0];   // bj + k < mj + 64
                             const unsigned jsh = __float_as_uint(pj.w);
-                            const int jb = (int)((jsh >> 8) & 0xffu);
+                            const int jb = (int)((jsh >> kJbShift) & kJbMask);
                             const float cj = fmaf(pj.z, pj.z, fmaf(pj.y, pj.y, pj.x * pj.x));
 #pragma unroll
                             for (int ch = 0; ch < NCHUNK; ++ch) {
@@ -816,7 +818,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     float4 sh;   // slot and shift again from LDS rather than held in registers
                     const int sl = locate(nxt, tid, sh);
                     if (ANA) s_je[tid] = shift_error(pf, sh);
-                    const float4 r = scan_rec(pf, sh, ((unsigned)tid << 8) + ((unsigned)tf << 16));
+                    const float4 r = scan_rec(pf, sh, ((unsigned)tid << kJbShift) + ((unsigned)tf << kTpShift));
                     s_js[tid] = r;
                     if (tid < 64) s_js[nxt.mj + tid] = r;
                     if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
@@ -940,6 +942,7 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
     if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
     if (s.excl && !fast) return false;   // exclusions: the int64 table path only
     if (sc.apw == 64 && sc.tile == 256) return true;
+    if (sc.tile == 512) return fast && sc.apw == 64;   // two or more cells per pass
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
@@ -974,7 +977,9 @@ static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st
 template <bool COUL, bool EXC>
 static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, size_t lds, hipStream_t st)
 {
-    if (sc.tile == 128) {
+    if (sc.tile == 512) {
+        launch_rl<COUL, false, false, false, 64, 512, EXC>(a, grid, lds, st);
+    } else if (sc.tile == 128) {
         if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 128, EXC>(a, grid, lds, st);
         else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 128, EXC>(a, grid, lds, st);
         else launch_rl<COUL, false, false, false, 64, 128, EXC>(a, grid, lds, st);
