@@ -403,6 +403,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     __shared__ float4 s_cshift[14];          // periodic image shift of each cell
     __shared__ int s_jslot[2][EN ? 1 : TJ];  // slot of each neighbour-tile atom (two passes)
     __shared__ double s_ered[EN ? NWAVE : 1][8];   // EN: per-wave sums
+    __shared__ float s_hb[TI / 64][6];       // home tile: per-wave bounding box (corner-relative)
+    __shared__ int s_wkeep[2][NWAVE];        // neighbour tile: atoms each wave keeps (two passes)
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -535,31 +537,114 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             // next pass is loaded into registers before a pass's pairs and written to LDS
             // after it, so its latency hides behind the pass
             NbrTile cur = find(obeg, 0);
+            // the home tile's bounding box in the scan's corner-relative coordinates (the
+            // exact minima and maxima of the scan's own home coordinates); a non-finite
+            // coordinate makes it the whole space
+            float blx = INFINITY, bly = INFINITY, blz = INFINITY;
+            float bhx = -INFINITY, bhy = -INFINITY, bhz = -INFINITY;
             for (int k = tid; k < mi; k += NTHR) {
-                s_i[k] = a.pos[hstart + itile + k];
+                const float4 p = a.pos[hstart + itile + k];
+                s_i[k] = p;
                 s_it[k] = a.type[hstart + itile + k];
                 if (EXC) {
                     const int io = a.orig[hstart + itile + k];
                     s_iom[k] = make_int2(io, (int)a.excl[io]);
                 }
+                const float xr = p.x - x0, yr = p.y - y0, zr = p.z - z0;
+                if (fabsf(xr) < 1e30f && fabsf(yr) < 1e30f && fabsf(zr) < 1e30f) {
+                    blx = fminf(blx, xr); bly = fminf(bly, yr); blz = fminf(blz, zr);
+                    bhx = fmaxf(bhx, xr); bhy = fmaxf(bhy, yr); bhz = fmaxf(bhz, zr);
+                } else {
+                    blx = bly = blz = -INFINITY;
+                    bhx = bhy = bhz = INFINITY;
+                }
             }
             if (!EN)
                 for (int k = tid; k < 3 * TI; k += NTHR) s_iacc[k] = 0ull;
+            // the first neighbour tile into registers
+            float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+            int tf = 0, of = 0;
             if (tid < cur.mj) {
                 float4 sh;
                 const int sl = locate(cur, tid, sh);
-                float4 p = a.pos[sl];
-                if (ANA) s_je[tid] = shift_error(p, sh);
-                const float4 r = scan_rec(p, sh, ((unsigned)tid << kJbShift) + ((unsigned)a.type[sl] << kTpShift));
-                s_js[tid] = r;
-                if (tid < 64) s_js[cur.mj + tid] = r;
-                if (!PAR) { p.x = p.x + sh.x; p.y = p.y + sh.y; p.z = p.z + sh.z; }
-                s_j[tid] = p;                   // .w: the scaled charge
-                if (!EN) s_jslot[0][tid] = sl;
-                if (EXC) s_jo[tid] = a.orig[sl];
+                pf = a.pos[sl];
+                tf = a.type[sl];
+                if (EXC) of = a.orig[sl];
+            }
+            if (tid < TI) {
+#pragma unroll
+                for (int m = 32; m > 0; m >>= 1) {
+                    blx = fminf(blx, __shfl_xor(blx, m)); bly = fminf(bly, __shfl_xor(bly, m));
+                    blz = fminf(blz, __shfl_xor(blz, m)); bhx = fmaxf(bhx, __shfl_xor(bhx, m));
+                    bhy = fmaxf(bhy, __shfl_xor(bhy, m)); bhz = fmaxf(bhz, __shfl_xor(bhz, m));
+                }
+                if (lane == 0) {
+                    s_hb[wave][0] = blx; s_hb[wave][1] = bly; s_hb[wave][2] = blz;
+                    s_hb[wave][3] = bhx; s_hb[wave][4] = bhy; s_hb[wave][5] = bhz;
+                }
             }
             if (!EN)
                 for (int k = tid; k < 3 * TJ; k += NTHR) s_jacc[k] = 0ull;
+            lds_barrier();
+            blx = s_hb[0][0]; bly = s_hb[0][1]; blz = s_hb[0][2];
+            bhx = s_hb[0][3]; bhy = s_hb[0][4]; bhz = s_hb[0][5];
+            for (int w = 1; w < (mi + 63) / 64; ++w) {
+                blx = fminf(blx, s_hb[w][0]); bly = fminf(bly, s_hb[w][1]); blz = fminf(blz, s_hb[w][2]);
+                bhx = fmaxf(bhx, s_hb[w][3]); bhy = fmaxf(bhy, s_hb[w][4]); bhz = fmaxf(bhz, s_hb[w][5]);
+            }
+            // (the box is the same on every lane: held in scalar registers)
+            auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+            blx = uni(blx); bly = uni(bly); blz = uni(blz);
+            bhx = uni(bhx); bhy = uni(bhy); bhz = uni(bhz);
+            // A neighbour image farther than rc from the home tile's box has no pair with
+            // it: the squared distance of its scan record to the box, compared with the
+            // scan's own threshold (rc^2 plus a margin far above this test's rounding and
+            // the corner-relative coordinates' error against the pair's own separation), drops
+            // it before the scan.  Tiles of the home cell keep every atom (the diagonal test
+            // needs their own indices); a NaN coordinate gives distance 0 and is kept.
+            auto near_box = [&](float4 r) {
+                const float ex = fmaxf(fmaxf(blx - r.x, r.x - bhx), 0.0f);
+                const float ey = fmaxf(fmaxf(bly - r.y, r.y - bhy), 0.0f);
+                const float ez = fmaxf(fmaxf(blz - r.z, r.z - bhz), 0.0f);
+                return !(fmaf(ez, ez, fmaf(ey, ey, ex * ex)) > rc2s);
+            };
+            // Stage the atoms of tile t held in registers (p, type, input index) that pass
+            // the box test: the test and a ballot before a barrier (the wave's count into
+            // s_wkeep[par]), the compacted records after it, at the atom's rank among the kept.
+            unsigned long long kmask = 0;
+            int ksl = 0;
+            float4 ksh = make_float4(0.f, 0.f, 0.f, 0.f);
+            auto keep_test = [&](const NbrTile &t, float4 p, int par) {
+                bool keep = false;
+                if (tid < t.mj) {
+                    ksl = locate(t, tid, ksh);
+                    keep = (t.o == 0) || near_box(scan_rec(p, ksh, 0u));
+                }
+                kmask = __builtin_amdgcn_ballot_w64(keep);
+                if (lane == 0) s_wkeep[par][wave] = (int)__popcll(kmask);
+            };
+            auto kept_count = [&](int par) {
+                int tot = 0;
+                for (int w = 0; w < NWAVE; ++w) tot += s_wkeep[par][w];
+                return __builtin_amdgcn_readfirstlane(tot);
+            };
+            auto stage_kept = [&](float4 p, int t, int o, int par, int mk) {
+                if (!((kmask >> lane) & 1ull)) return;
+                int k = (int)lane_rank(kmask);
+                for (int w = 0; w < wave; ++w) k += s_wkeep[par][w];
+                if (ANA) s_je[k] = shift_error(p, ksh);
+                const float4 r = scan_rec(p, ksh, ((unsigned)k << kJbShift) + ((unsigned)t << kTpShift));
+                s_js[k] = r;
+                if (k < 64) s_js[mk + k] = r;
+                if (!PAR) { p.x = p.x + ksh.x; p.y = p.y + ksh.y; p.z = p.z + ksh.z; }
+                s_j[k] = p;                     // .w: the scaled charge
+                if (!EN) s_jslot[par][k] = ksl;
+                if (EXC) s_jo[k] = o;
+            };
+            keep_test(cur, pf, 0);
+            lds_barrier();
+            int mk = kept_count(0);   // atoms of the current tile kept in LDS
+            stage_kept(pf, tf, of, 0, mk);
             lds_barrier();
             int jpar = 0;
 
@@ -579,8 +664,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     }
                 }
                 const NbrTile nxt = (cur.oe == cur.o + 1) ? find(cur.o, cur.jt + TJ) : find(cur.oe, 0);
-                float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
-                int tf = 0, of = 0;
                 if (tid < nxt.mj) {
                     float4 sh;
                     const int sl = locate(nxt, tid, sh);
@@ -589,7 +672,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     if (EXC) of = a.orig[sl];
                 }
                 const bool self = (cur.o == 0);
-                const int jtile = cur.jt, mj = cur.mj;
+                const int jtile = cur.jt, mj = mk;   // the kept atoms of the tile
                 const float4 csh = s_cshift[cur.o];   // the tile's shift (single-cell tiles)
                 const float sx = csh.x, sy = csh.y, sz = csh.z;
 
@@ -714,9 +797,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
 
                 // the wave's queue holds entries q[0, tail), tail < 64 between tests
                 unsigned tail = 0;
-                const int bj = il % mj;
+                const int bj = mj > 0 ? il % mj : 0;
                 const int nsteps = (mj + P - 1) / P;
-                for (int cg0 = 0; cg0 < mi; cg0 += NCHUNK * APW) {
+                for (int cg0 = 0; mj > 0 && cg0 < mi; cg0 += NCHUNK * APW) {
                     if (self && itile + cg0 >= jtile + mj) break;   // rest is above the diagonal
                     const int nch = min(NCHUNK, (mi - cg0 + APW - 1) / APW);
                     // this lane's home atoms (far away past the tile: r2 = inf never
@@ -790,10 +873,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     if ((unsigned)lane < tail) eval(q[lane]);
                     __builtin_amdgcn_wave_barrier();
                 }
+                keep_test(nxt, pf, jpar ^ 1);
                 lds_barrier();
+                const int mkn = kept_count(jpar ^ 1);
                 // flush neighbour partial forces (D/Force_Write_Back_Controller.v:370-429)
                 // and clear the accumulators for the next pass; the atomics stay in flight
-                const int nclr = 3 * max(mj, nxt.mj);
+                const int nclr = 3 * max(mj, mkn);
                 if (EN) {
                     // (no forces in the energy pass)
                 } else if (F32) {
@@ -814,21 +899,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         s_jacc[k] = 0ull;
                     }
                 }
-                if (tid < nxt.mj) {
-                    float4 sh;   // slot and shift again from LDS rather than held in registers
-                    const int sl = locate(nxt, tid, sh);
-                    if (ANA) s_je[tid] = shift_error(pf, sh);
-                    const float4 r = scan_rec(pf, sh, ((unsigned)tid << kJbShift) + ((unsigned)tf << kTpShift));
-                    s_js[tid] = r;
-                    if (tid < 64) s_js[nxt.mj + tid] = r;
-                    if (!PAR) { pf.x = pf.x + sh.x; pf.y = pf.y + sh.y; pf.z = pf.z + sh.z; }
-                    s_j[tid] = pf;
-                    if (!EN) s_jslot[jpar ^ 1][tid] = sl;
-                    if (EXC) s_jo[tid] = of;
-                }
+                stage_kept(pf, tf, of, jpar ^ 1, mkn);
                 jpar ^= 1;
                 lds_barrier();
                 cur = nxt;
+                mk = mkn;
             }
             // flush reference-particle forces (D/Partial_Force_Acc.v:65-129)
             if (EN) {
