@@ -355,13 +355,16 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
         s.rc2_64 = rcd * rcd;
         s.rc2_hi = nextafterf((float)(s.rc2_64 + m), INFINITY);
         s.rc2_lo = nextafterf((float)(s.rc2_64 - m), 0.0f);
-        // The candidate scan evaluates r2 = (c_i + c_j) - 2 p_i.p_j on coordinates relative
-        // to the home cell's corner, each within B = 2.1 E of it (E: the largest cell edge;
-        // the home cell spans [0, E), its neighbours and their images [-E, 2E)).  Its error
-        // against the pair's own r2 is at most u (54 B^2 + 7 B rc + 6 rc^2): c = |p|^2 and the
-        // sum c_i + c_j within 2u 3B^2 each and u 6B^2, three fused steps whose partial sums
-        // stay within 12 B^2, the corner subtractions within u B per coordinate, and the
-        // pair's own rounding of dx and r2.  The scan passes r2 <= rc^2 + twice that bound.
+        // The candidate scan tests c_j - 2 p_i.p_j <= fl(T - c_i) (T the threshold below) on
+        // coordinates relative to the home cell's corner, each within B = 2.1 E of it (E: the
+        // largest cell edge; the home cell spans [0, E), its neighbours and their images
+        // [-E, 2E)).  Against the pair's own r2 <= T the test errs by at most
+        // u (54 B^2 + 7 B rc + 6 rc^2): c = |p|^2 within 2u 3B^2 for each atom, the
+        // threshold's subtraction within u 3B^2, three fused steps whose partial sums stay
+        // within 9 B^2 (27 u B^2), the corner subtractions within u B per coordinate, and the
+        // pair's own rounding of dx and r2.  The staging filter (squared distance of a
+        // neighbour to the home tile's box against T) errs by less.  The scan passes with
+        // T = rc^2 + twice that bound.
         double emax = 0.0;
         for (const SysGeom &G : s.h_geom)
             for (int d = 0; d < 3; ++d) emax = std::max(emax, (double)G.box[d] / G.nc[d]);

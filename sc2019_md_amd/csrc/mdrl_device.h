@@ -201,12 +201,14 @@ __device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const
 {
     const unsigned wi = (__float_as_uint(r2) >> 15) - (unsigned)(off * 256 + win_lo);
     const bool inw = wi < (unsigned)nwin;
-    const int li = inw ? (int)wi : 0;
+    // outside the window: the last entry (in bounds; its values are replaced below)
+    const unsigned li = min(wi, (unsigned)max(nwin - 1, 0));
     if (STR == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(s_tab + (li << 2));
         c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
     } else {
-        const float *e = s_tab + __umul24((unsigned)li, 6u);   // 24-bit multiply: full rate
+        // 24-bit multiply-add of the byte offset: one full-rate instruction
+        const float *e = reinterpret_cast<const float *>(reinterpret_cast<const char *>(s_tab) + __umul24(li, 24u));
         const float2 u = *reinterpret_cast<const float2 *>(e);
         const float2 v = *reinterpret_cast<const float2 *>(e + 2);
         const float2 w = *reinterpret_cast<const float2 *>(e + 4);

@@ -334,6 +334,7 @@ struct ForceArgs {
     float rc2;                // the reference's FP32 cutoff test r2 <= rc2
     float rc2_scan;           // the candidate scan's test on its own r2 (rc2 plus its error bound)
     float rc2_lo;             // analytic mode: r2 above this is decided in FP64
+    float fk_fast;            // |F 2^32| below this: every |F 2^32 d| of a pair within rc is < 2^51
     double rc2_64;            // analytic mode: the exact test r^2 <= rc * rc
     const float *tab;
     const float *etab;        // energy pass: c0/c1 of r^-12, r^-6, r^-1, 6 floats per entry
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
     unsigned *q = s_q[wave];
     unsigned flag = 0;
-    unsigned long long npairs = 0, ncand = 0, ndrop = 0;
+    unsigned long long npairs = 0, ncand = 0;   // pairs evaluated and kept; candidate tests
     const float rc2s = a.rc2_scan;
 
     // One block of bonded terms: forces accumulate in the home-tile accumulators
@@ -750,12 +751,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         const int d = s_jo[jb] - om.x + 16;
                         drop = drop || ((unsigned)d < 32u && (((unsigned)om.y >> d) & 1u));
                     }
-                    ndrop += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(drop));
-                    F = drop ? 0.0f : F;
+                    const bool keep = !drop;
+                    npairs += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(keep));
+                    F = keep ? F : 0.0f;
                     if (EN) {
                         // the pair's energy, its energy at rc and its virial F d (x) d
-                        Ee = drop ? 0.0f : Ee;
-                        Ue = drop ? 0.0f : Ue;
+                        Ee = keep ? Ee : 0.0f;
+                        Ue = keep ? Ue : 0.0f;
                         eacc[0] += (double)Ee;
                         eacc[1] += (double)Ue;
                         if constexpr (EN == 1) {
@@ -776,15 +778,21 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         atomic_add_wg(&s_jacc_f[3 * jb + 1], -fy);
                         atomic_add_wg(&s_jacc_f[3 * jb + 2], -fz);
                     } else {
-                        // rint(F*d * 2^32): F is scaled first (exact, and the products then
-                        // carry the bits of F*d times 2^32); a force too large to scale, or
-                        // not finite, takes the unscaled conversion
-                        long long kx, ky, kz;
-                        if (fabsf(F) < 0x1p95f) {
-                            const float Fk = F * 4294967296.0f;
-                            fix3_scaled(Fk * dx, Fk * dy, Fk * dz, kx, ky, kz, flag);
-                        } else {
-                            fix3(F * dx, F * dy, F * dz, kx, ky, kz, flag);
+                        // rint(F*d * 2^32): F is scaled first (exact while finite, and the
+                        // products then carry the bits of F*d times 2^32).  With |F 2^32| below
+                        // fk_fast every |F 2^32 d| of an evaluated pair (|d| <= rc) is below
+                        // 2^51, where one FP64 add of 1.5 * 2^52 rounds to the integer; other
+                        // lanes (rare) take the general conversion:
+                        // scaled, or unscaled for a force too large to scale, with saturation
+                        // and NaN flags.  Both give rint(F*d * 2^32) wherever that is in range.
+                        const float Fk = F * 4294967296.0f;
+                        const float xk = Fk * dx, yk = Fk * dy, zk = Fk * dz;
+                        long long kx = __double_as_longlong((double)xk + 6755399441055744.0) - 0x4338000000000000LL;
+                        long long ky = __double_as_longlong((double)yk + 6755399441055744.0) - 0x4338000000000000LL;
+                        long long kz = __double_as_longlong((double)zk + 6755399441055744.0) - 0x4338000000000000LL;
+                        if (!(fabsf(Fk) < a.fk_fast)) {
+                            if (fabsf(F) < 0x1p95f) fix3_scaled(xk, yk, zk, kx, ky, kz, flag);
+                            else fix3(F * dx, F * dy, F * dz, kx, ky, kz, flag);
                         }
                         atomic_add_wg(&s_iacc[3 * ia + 0], (unsigned long long)kx);
                         atomic_add_wg(&s_iacc[3 * ia + 1], (unsigned long long)ky);
@@ -802,14 +810,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 for (int cg0 = 0; mj > 0 && cg0 < mi; cg0 += NCHUNK * APW) {
                     if (self && itile + cg0 >= jtile + mj) break;   // rest is above the diagonal
                     const int nch = min(NCHUNK, (mi - cg0 + APW - 1) / APW);
-                    // this lane's home atoms (far away past the tile: r2 = inf never
+                    // this lane's home atoms (lanes past the tile: a threshold nothing
                     // passes) and, in the home cell, the neighbour index each pair must
                     // exceed so that a pair is taken once as (lower, higher)
-                    // The scan's r2 = (c_i + c_j) - 2 p_i . p_j on corner-relative coordinates
-                    // (c = |p|^2): each home atom holds -2 p_i and c_i, lanes past the tile a
-                    // c_i that never passes.  It differs from the reference r2 by less than the
-                    // margin folded into rc2_scan; evaluation applies the reference test.
-                    float hx2[NCHUNK], hy2[NCHUNK], hz2[NCHUNK], hc[NCHUNK];
+                    // The scan tests c_j - 2 p_i . p_j <= rc2_scan - c_i on corner-relative
+                    // coordinates (c = |p|^2): each home atom holds -2 p_i and its threshold
+                    // rc2_scan - c_i.  The test differs from the reference r2 <= rc2 by less
+                    // than the margin folded into rc2_scan; evaluation applies the reference test.
+                    float hx2[NCHUNK], hy2[NCHUNK], hz2[NCHUNK], th[NCHUNK];
                     unsigned eb[NCHUNK];
 #pragma unroll
                     for (int ch = 0; ch < NCHUNK; ++ch) {
@@ -818,9 +826,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                             const float4 p = s_i[ia];
                             const float xr = p.x - x0, yr = p.y - y0, zr = p.z - z0;
                             hx2[ch] = -2.0f * xr; hy2[ch] = -2.0f * yr; hz2[ch] = -2.0f * zr;
-                            hc[ch] = fmaf(zr, zr, fmaf(yr, yr, xr * xr));
+                            th[ch] = rc2s - fmaf(zr, zr, fmaf(yr, yr, xr * xr));
                         } else {
-                            hx2[ch] = 0.0f; hy2[ch] = 0.0f; hz2[ch] = 0.0f; hc[ch] = 1e30f;
+                            hx2[ch] = 0.0f; hy2[ch] = 0.0f; hz2[ch] = 0.0f; th[ch] = -INFINITY;
                         }
                         eb[ch] = (unsigned)ia + ((unsigned)(((ia < mi) ? s_it[ia] : 0) * a.ntypes) << kTpShift);
                     }
@@ -839,13 +847,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
 #pragma unroll
                             for (int ch = 0; ch < NCHUNK; ++ch) {
                                 if (ch < nch) {
-                                    const float r2 = fmaf(hz2[ch], pj.z, fmaf(hy2[ch], pj.y, fmaf(hx2[ch], pj.x, hc[ch] + cj)));
-                                    const bool pass = kv && (r2 <= rc2s) && (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
+                                    const float v = fmaf(hz2[ch], pj.z, fmaf(hy2[ch], pj.y, fmaf(hx2[ch], pj.x, cj)));
+                                    const bool pass = kv && (v <= th[ch]) && (!SELF || jb > dlim + (int)(eb[ch] & 0xffu));
                                     const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
                                     // slot = tail + the lane's rank among the passing lanes
+                                    unsigned *qt = q + tail;
                                     if (pass)
-                                        q[__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, tail))] = eb[ch] + jsh;
+                                        qt[__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = eb[ch] + jsh;
                                     tail += (unsigned)__popcll(m);
                                     if (tail >= 64u) {
                                         // evaluate the first 64, move the rest (< 64) to the front
@@ -856,7 +865,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                                         __builtin_amdgcn_wave_barrier();
                                         if ((unsigned)lane < rest) q[lane] = mv;
                                         __builtin_amdgcn_wave_barrier();
-                                        npairs += 64u;
                                         tail = rest;
                                     }
                                 }
@@ -867,7 +875,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     else run_steps(std::false_type{});
                     ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
                 }
-                npairs += tail;
                 if (tail) {
                     __builtin_amdgcn_wave_barrier();
                     if ((unsigned)lane < tail) eval(q[lane]);
@@ -951,7 +958,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     if (EN) return;
     // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
-        s_stat[wave][0] = npairs - ndrop;
+        s_stat[wave][0] = npairs;
         s_stat[wave][1] = ncand * 64ull;
     }
     lds_barrier();
@@ -1082,6 +1089,10 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.tasks = s.sched.persistent ? s.tasks : nullptr;
     a.rc2 = s.rc2;
     a.rc2_scan = ana ? s.rc2_scan_ana : s.rc2_scan;
+    // an evaluated pair has r2 <= rc2 (rc2_hi in analytic mode, below rc2_scan_ana), so each
+    // |d| component is at most sqrt(rc2_scan_ana) (1 + 2^-22); 1.001 covers that and the
+    // rounding of the quotient
+    a.fk_fast = (float)(ldexp(1.0, 51) / (1.001 * sqrt((double)std::max(s.rc2_scan_ana, s.rc2_scan))));
     a.rc2_lo = s.rc2_lo;
     a.rc2_64 = s.rc2_64;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
