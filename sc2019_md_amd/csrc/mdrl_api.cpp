@@ -259,12 +259,14 @@ static int table_segment(float r2, int off, int nseg)
 
 // Force tasks (home cell x offset group), largest estimated cost first: the persistent
 // kernel drains them from a counter, so the long tasks of dense small boxes start early.
+// This first order uses each system's mean cell population; every binning then rewrites
+// it from the actual populations (k_scan).
 static int build_tasks(md_rl_ctx *c)
 {
     DeviceSystem &s = c->s;
     const int g = s.sched.groups;
-    static const double w1[1] = {13.5}, w2[2] = {6.5, 7.0}, w4[4] = {3.5, 4.0, 3.0, 3.0};
-    const double *w = g == 1 ? w1 : g == 2 ? w2 : w4;
+    float w[4];
+    group_weights(g, w);
     std::vector<std::pair<double, int>> cost;
     cost.reserve((size_t)s.ncells * g);
     for (const SysGeom &G : s.h_geom) {
@@ -706,7 +708,9 @@ static int upload(md_rl_ctx *c)
         return hip_fail(e, "hipMemcpyAsync");
     int rc;
     if ((rc = upload_topology(c))) return rc;
-    if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream))) return rc;
+    if ((rc = launch_bin_initial(s, c->stream)) || (rc = launch_rebin(s, c->stream)) ||
+        (rc = launch_task_order(s, c->stream)))
+        return rc;
     s.first_step = true;
     drop_graphs(c);
     if ((rc = launch_zero_forces(s, c->stream)) || (rc = forces(c, c->stream))) return rc;
@@ -1313,7 +1317,14 @@ extern "C" int md_rl_set_schedule(md_rl_ctx *c, const md_rl_schedule *sc)
     const bool regroup = s.groups != c->s.sched.groups;
     c->s.sched = s;
     drop_graphs(c);
-    return regroup ? build_tasks(c) : MDRL_OK;
+    if (regroup) {
+        const int rc = build_tasks(c);
+        if (rc) return rc;
+    }
+    // the task order of the new schedule from the current cell populations (once the
+    // particles are binned; every kOrderEvery-th binning rederives it)
+    if (!c->have_particles || c->dirty) return MDRL_OK;   // the upload orders them
+    return launch_task_order(c->s, c->stream);
 }
 
 extern "C" int md_rl_get_schedule(const md_rl_ctx *c, md_rl_schedule *sc)

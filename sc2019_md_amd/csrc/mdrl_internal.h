@@ -130,7 +130,7 @@ struct DeviceSystem {
     float2 *pair_ab_e = nullptr;              // ntypes^2 (eps sig^12, eps sig^6)
     double *hk = nullptr;                     // ntypes: half-kick factor
     double *mass = nullptr;                   // ntypes
-    unsigned long long *counters = nullptr;   // kCounterWords: [2] flags [3] max cell, then
+    unsigned long long *counters = nullptr;   // kCounterWords: [2] flags [3] max cell [4] binnings, then
                                               // kStatSlots x {pairs, candidates}, then the
                                               // kQueues work counters
     double *cell_red = nullptr;               // ncells x 11: PE, virial, PE at rc per home cell
@@ -159,6 +159,7 @@ struct DeviceSystem {
 // Kernel launchers (mdrl_system.hip).
 int launch_bin_initial(DeviceSystem &s, hipStream_t st);   // cell ids of the current slots
 int launch_rebin(DeviceSystem &s, hipStream_t st);         // scan, scatter, sort, gather
+int launch_task_order(DeviceSystem &s, hipStream_t st);    // force-task order from cell_start
 int launch_force(DeviceSystem &s, hipStream_t st);
 int launch_update(DeviceSystem &s, float dt, hipStream_t st);
 int launch_minimize(DeviceSystem &s, double gamma, double dmax, hipStream_t st);
@@ -168,6 +169,7 @@ int launch_zero_forces(DeviceSystem &s, hipStream_t st);   // forces and pair co
 enum { kEnergyKinetic = 0, kEnergyPotential = 1, kEnergyVirial = 2 };
 int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st, int mode);
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc);
+void group_weights(int ngroups, float *w);       // offset groups' shares of a cell's cost
 // Summation hook: {x, y, z, q} records in input order out of the slots (clearing the
 // provider's forces f, n x 3, and energies e, nsys), and the provider's forces back in.
 int launch_lr_export(DeviceSystem &s, const float *q, float4 *xq, float *f, double *e, hipStream_t st);

@@ -1008,6 +1008,15 @@ static BondedArgs bonded_args(DeviceSystem &s)
 }
 
 // Offset groups: the home cell is half the work of a full neighbour cell.
+// Shares of a home cell's cost taken by each offset group (the self cell counts about half
+// a neighbour cell).
+void group_weights(int ngroups, float *w)
+{
+    static const float w1[1] = {13.5f}, w2[2] = {6.5f, 7.0f}, w4[4] = {3.5f, 4.0f, 3.0f, 3.0f};
+    const float *v = ngroups == 1 ? w1 : ngroups == 2 ? w2 : w4;
+    for (int k = 0; k < 4; ++k) w[k] = k < ngroups ? v[k] : 0.0f;
+}
+
 static void offset_groups(int ngroups, int *gstart)
 {
     static const int g1[2] = {0, 14};
@@ -1154,18 +1163,148 @@ __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const in
     run_add(count + count_copy() * (size_t)ncells, c, s < n);
 }
 
+// Force-task order for the persistent schedule, written by k_scan.  A
+// task's cost is estimated from the populations of its home cell and half-shell
+// neighbours: the candidate scan costs a pass per 64 home atoms, the evaluation grows with
+// the home atoms (cost = (ceil(n/64) + 0.7 n/64) (n/2 + neighbours), times the offset
+// group's share).  Tasks are counting-sorted by log2(cost) in 1/32-octave bins, largest
+// first; populations change slowly, so the order is rederived at every kOrderEvery-th
+// binning (counted in counters[4]).  When the tasks fit the resident grid (one per
+// workgroup, two workgroups per CU, workgroups b and b + cus sharing a CU), the largest go
+// to the CUs that hold one workgroup and each remaining CU pairs a large task with a small
+// one.  The order decides only which workgroup runs a task: forces are the same for any
+// order.
+struct TaskOrder {
+    const SysGeom *geom;
+    const int *cell_sys;
+    int *tasks;          // the persistent schedule's task list
+    int ngroups;
+    float w[4];          // the offset groups' shares of a cell's cost
+    int pair_cus;        // CUs for the pairing placement, 0: plain order
+};
+constexpr int kOrderBins = 1024, kOrderEvery = 16;
+// cells whose populations the order pass reads from LDS (8 per thread of k_scan); larger
+// batches read them from start[]
+constexpr int kOrderCells = 8192;
+
+template <typename Count>
+__device__ __forceinline__ float task_cost(Count cnt, const SysGeom &G, int c)
+{
+    const int k = c - G.cell_off;
+    const int hz = k % G.nc[2], hy = (k / G.nc[2]) % G.nc[1], hx = k / (G.nc[1] * G.nc[2]);
+    const int nh = cnt(c);
+    float nb = 0.5f * (float)nh;
+    for (int o = 1; o < 14; ++o) {
+        int x = hx + c_hs[o][0], y = hy + c_hs[o][1], z = hz + c_hs[o][2];
+        x = x < 0 ? x + G.nc[0] : (x >= G.nc[0] ? x - G.nc[0] : x);
+        y = y < 0 ? y + G.nc[1] : (y >= G.nc[1] ? y - G.nc[1] : y);
+        z = z < 0 ? z + G.nc[2] : (z >= G.nc[2] ? z - G.nc[2] : z);
+        nb += (float)cnt(G.cell_off + (x * G.nc[1] + y) * G.nc[2] + z);
+    }
+    return ((float)((nh + 63) / 64) + 0.7f * (float)nh * (1.0f / 64.0f)) * nb;
+}
+__device__ __forceinline__ int cost_bin(float cost)
+{
+    // descending: bin 0 holds the largest costs
+    const int b = (int)(log2f(cost + 1.0f) * 32.0f);
+    return kOrderBins - 1 - min(max(b, 0), kOrderBins - 1);
+}
+
+// The force-task order of TaskOrder: one workgroup of 1024 threads; s_sum and s_wsum are
+// 1024- and 16-int LDS scratch, s_cnt the populations of the first kOrderCells cells (when
+// ncells <= kOrderCells; else they are read from start[]).
+__device__ __forceinline__ void order_tasks(int ncells, const int *start, const TaskOrder &to, int *s_sum,
+                                            int *s_wsum, const int *s_cnt)
+{
+    const int t = threadIdx.x;
+    const bool lds = ncells <= kOrderCells;
+    auto cnt = [&](int c) { return lds ? s_cnt[c] : start[c + 1] - start[c]; };
+    const int g = to.ngroups;
+    const int nt = ncells * g, cus = to.pair_cus;
+    const int npair = nt - cus, nsingle = cus - npair;
+    const bool pairing = cus > 0 && nt > cus && nt <= 2 * cus;
+    s_sum[t] = 0;
+    // costs of this thread's cells c = t + 1024 k, kept in registers when the cells fit LDS
+    constexpr int KC = kOrderCells / 1024;
+    float cst[KC];
+    if (lds) {
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int c = t + 1024 * k;
+            cst[k] = c < ncells ? task_cost(cnt, to.geom[to.cell_sys[c]], c) : 0.0f;
+        }
+    }
+    __syncthreads();
+    // pass 0 counts the tasks per bin, pass 1 places them
+    for (int pass = 0; pass < 2; ++pass) {
+        auto visit = [&](int c, float cost) {
+            for (int q = 0; q < g; ++q) {
+                const int bin = cost_bin(cost * to.w[q]);
+                if (pass == 0) {
+                    atomicAdd(&s_sum[bin], 1);
+                } else {
+                    const int pos = atomicAdd(&s_sum[bin], 1);
+                    int at = pos;
+                    if (pairing)
+                        at = pos < nsingle ? npair + pos : (pos < cus ? pos - nsingle : cus + (nt - 1 - pos));
+                    to.tasks[at] = c * g + q;
+                }
+            }
+        };
+        if (lds) {
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+                if (t + 1024 * k < ncells) visit(t + 1024 * k, cst[k]);
+        } else {
+            for (int c = t; c < ncells; c += 1024) visit(c, task_cost(cnt, to.geom[to.cell_sys[c]], c));
+        }
+        __syncthreads();
+        if (pass == 0) {
+            // exclusive scan of the bin counts: first position of each bin
+            const int lane = t & 63, w = t >> 6;
+            const int v = s_sum[t];
+            int inc = v;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(inc, off);
+                if (lane >= off) inc += u;
+            }
+            if (lane == 63) s_wsum[w] = inc;
+            __syncthreads();
+            int before = 0;
+            for (int k = 0; k < w; ++k) before += s_wsum[k];
+            s_sum[t] = before + inc - v;
+            __syncthreads();
+        }
+    }
+}
+
+// The task order alone (after a schedule change), from the current cell_start.
+__global__ __launch_bounds__(1024) void k_task_order(int ncells, const int *start, TaskOrder to)
+{
+    __shared__ int s_sum[1024];
+    __shared__ int s_wsum[16];
+    __shared__ int s_cnt[kOrderCells];
+    if (ncells <= kOrderCells)
+        for (int c = threadIdx.x; c < ncells; c += 1024) s_cnt[c] = start[c + 1] - start[c];
+    order_tasks(ncells, start, to, s_sum, s_wsum, s_cnt);
+}
+
 // Exclusive scan of the next-step counts (kCountCopies copies per cell, summed) into
 // cell_start, and the start of each copy's share of each cell into the fill pointers;
 // clears the counts; records the largest cell; clears the pair counters and the work
 // queues of the force evaluation that follows.  One workgroup of 1024 threads walks the
 // cells in tiles of 1024, one cell per thread, so every load and store is coalesced and
-// each thread's kCountCopies loads are independent.
+// each thread's kCountCopies loads are independent.  Then, for the persistent schedule,
+// the force-task order (above).
 __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *start, int *fill,
-                                               unsigned long long *counters)
+                                               unsigned long long *counters, TaskOrder to)
 {
     __shared__ int s_sum[1024];
     __shared__ int s_wsum[16];
     __shared__ int s_run;
+    __shared__ int s_cnt[kOrderCells];   // cell populations for the task order
+    __shared__ int s_order;
     const int t = threadIdx.x;
     if (t == 0) s_run = 0;
     __syncthreads();
@@ -1180,6 +1319,7 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
             tot += v[k];
         }
         mx = max(mx, tot);
+        if (c < ncells && c < kOrderCells) s_cnt[c] = tot;
         // inclusive scan of the tile: within each wave by shuffles, then across the 16
         // wave totals
         const int lane = t & 63, w = t >> 6;
@@ -1220,6 +1360,12 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
         counters[3] = (unsigned long long)s_sum[0];
     }
     if (t < 2 * kStatSlots + kQueues) counters[8 + t] = 0ull;
+    if (!to.tasks) return;
+    // the task order, at every kOrderEvery-th binning
+    if (t == 0) s_order = (int)(counters[4]++ % kOrderEvery) == 0;
+    __threadfence();    // start[] written and visible to the workgroup
+    __syncthreads();
+    if (s_order) order_tasks(ncells, start, to, s_sum, s_wsum, s_cnt);
 }
 
 // Each slot takes the next place of its cell's share for the copy its block counted into
@@ -1288,10 +1434,31 @@ int launch_bin_initial(DeviceSystem &s, hipStream_t st)
     return MDRL_OK;
 }
 
+static TaskOrder task_order_args(const DeviceSystem &s)
+{
+    TaskOrder to{};
+    to.geom = s.geom;
+    to.cell_sys = s.cell_sys;
+    to.tasks = s.tasks;
+    to.ngroups = s.sched.groups;
+    group_weights(to.ngroups, to.w);
+    // the pairing placement assumes the resident grid of the launch: range-limited tasks only
+    to.pair_cus = s.nblocks == 0 ? s.cu_count : 0;
+    return to;
+}
+
+int launch_task_order(DeviceSystem &s, hipStream_t st)
+{
+    if (!s.tasks || s.ncells == 0) return MDRL_OK;
+    hipLaunchKernelGGL(k_task_order, dim3(1), dim3(1024), 0, st, s.ncells, s.cell_start, task_order_args(s));
+    MDRL_HIP(hipGetLastError());
+    return MDRL_OK;
+}
+
 int launch_rebin(DeviceSystem &s, hipStream_t st)
 {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.ncells, s.cell_next, s.cell_start,
-                       s.cell_fill, s.counters);
+                       s.cell_fill, s.counters, task_order_args(s));
     if (s.n > 0) {
         hipLaunchKernelGGL(k_scatter, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.cell,
                            s.cell_fill, s.perm, s.ncells);
