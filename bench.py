@@ -403,6 +403,15 @@ def precision_leg(s, relax, relax_step, nve_steps, dt_list):
     return out, relaxed, {"int64": fix, "fp32": f32a}
 
 
+def tune_or_pin(g, pinned):
+    """The autotuner's schedule (through its cache), or the one given as "apw,tile,groups,
+    persistent": (schedule, ms of the winner or None, cache hit)."""
+    if not pinned:
+        return g.autotune(reps=3, cache=True)
+    g.set_schedule(*[int(v) for v in pinned.split(",")])
+    return g.schedule(), None, False
+
+
 def bench_batch(args, which):
     """A batched workload as one graph-captured batch: the 512 uneven LJ + Coulomb replicas
     (which = 4), or the bonded workload, 128 replicas of 64 chains x 48 beads with bonded
@@ -417,7 +426,7 @@ def bench_batch(args, which):
     relax = args.relax if args.relax else 0
     if relax:
         b.minimize(relax, args.relax_step, 1e-4)
-    sched, tune_ms, tune_hit = b.autotune(reps=3, cache=True)
+    sched, tune_ms, tune_hit = tune_or_pin(b, args.schedule)
     b.step(args.warmup)
     pe0, ke0, _ = b.energy()
     b.set_graph(bool(args.graph))
@@ -533,6 +542,8 @@ def main():
                     help="NVE steps of the precision leg (workload 3)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one device copy of this many bytes (traffic counter calibration)")
+    ap.add_argument("--schedule", default=None,
+                    help="atoms_per_wave,tile_depth,offset_groups,persistent instead of the autotuner's choice")
     ap.add_argument("--long-range-field", action="store_true",
                     help="attach a summation-hook provider (a uniform field on the charges, copied "
                          "from a device buffer each evaluation) for the steps, to time the hook")
@@ -562,7 +573,7 @@ def main():
     relaxed = g.particles()
     fix_relaxed = g.forces_fixed()     # the forces the timed steps start from
 
-    sched, tune_ms, tune_hit = g.autotune(reps=3, cache=True)
+    sched, tune_ms, tune_hit = tune_or_pin(g, args.schedule)
     field_buf = None
     if args.long_range_field:
         # F_i = q_i E with E = (0.8, -0.3, 0.45) kJ/mol/A/e, input order
