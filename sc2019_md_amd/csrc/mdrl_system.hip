@@ -1163,21 +1163,20 @@ __global__ void k_assign(int n, const float4 *pos, const SysGeom *geom, const in
     run_add(count + count_copy() * (size_t)ncells, c, s < n);
 }
 
-// Force-task order for the persistent schedule, written by k_scan.  A
-// task's cost is estimated from the populations of its home cell and half-shell
-// neighbours: the candidate scan costs a pass per 64 home atoms, the evaluation grows with
-// the home atoms (cost = (ceil(n/64) + 0.7 n/64) (n/2 + neighbours), times the offset
-// group's share).  Tasks are counting-sorted by log2(cost) in 1/32-octave bins, largest
-// first; populations change slowly, so the order is rederived at every kOrderEvery-th
-// binning (counted in counters[4]).  When the tasks fit the resident grid (one per
-// workgroup, two workgroups per CU, workgroups b and b + cus sharing a CU), the largest go
-// to the CUs that hold one workgroup and each remaining CU pairs a large task with a small
-// one.  The order decides only which workgroup runs a task: forces are the same for any
-// order.
+// Force-task order for the persistent schedule when its tasks fit the resident grid (one
+// task per workgroup, two workgroups per CU, workgroups b and b + cus sharing a CU),
+// written by k_scan.  A task's cost is estimated from the populations of its home cell and
+// half-shell neighbours: the candidate scan costs a pass per 64 home atoms, the evaluation
+// grows with the home atoms (cost = (ceil(n/64) + 0.7 n/64) (n/2 + neighbours), times the
+// offset group's share).  Tasks are counting-sorted by log2(cost) in 1/32-octave bins,
+// largest first; populations change slowly, so the order is rederived at every
+// kOrderEvery-th binning (counted in counters[4]).  The largest tasks go to the CUs that
+// hold one workgroup, and each other CU pairs a large task with a small one.  The order
+// decides only which workgroup runs a task: forces are the same for any order.
 struct TaskOrder {
     const SysGeom *geom;
     const int *cell_sys;
-    int *tasks;          // the persistent schedule's task list
+    int *tasks;          // the persistent schedule's task list (NULL: keep the host's order)
     int ngroups;
     float w[4];          // the offset groups' shares of a cell's cost
     int pair_cus;        // CUs for the pairing placement, 0: plain order
@@ -1282,6 +1281,7 @@ __device__ __forceinline__ void order_tasks(int ncells, const int *start, const 
 // The task order alone (after a schedule change), from the current cell_start.
 __global__ __launch_bounds__(1024) void k_task_order(int ncells, const int *start, TaskOrder to)
 {
+    if (!to.tasks) return;
     __shared__ int s_sum[1024];
     __shared__ int s_wsum[16];
     __shared__ int s_cnt[kOrderCells];
@@ -1378,12 +1378,25 @@ __global__ void k_scatter(int n, const int *cell, int *fill, int *perm, int ncel
     if (s < n) perm[k] = s;
 }
 
+// Comparator t of round (k, j) of a bitonic sorting network over N = 2^p keys, all
+// comparators ascending: the first round of each block size k compares an element with its
+// mirror in the block, the others with the element j further.  Keys past the m real ones
+// count as +infinity and never move, so comparators that reach them are skipped.
+__device__ __forceinline__ void bitonic_pair(int t, int k, int j, int &lo, int &hi)
+{
+    const int blk = t / j, off = t - blk * j;
+    if (j == (k >> 1)) { lo = blk * k + off; hi = blk * k + k - 1 - off; }
+    else { lo = blk * 2 * j + off; hi = lo + j; }
+}
+
 // One workgroup per cell: orders the cell's slots by input index and moves their particle
 // data into the new slots, clearing the forces and recording each input index's slot.
-// A cell larger than the LDS capacity keeps the scatter order and raises
-// MDRL_F_CELL_OVERFLOW (forces stay correct in int64 mode).
-constexpr int SORT_CAP = 2048;
-__global__ __launch_bounds__(256) void k_cell_gather(const int *start, const int *perm, const float4 *pos,
+// Up to 256 atoms each thread ranks its atom against the cell in LDS; up to SORT_CAP a
+// bitonic network sorts (input index, slot) pairs in LDS; a larger cell runs the same
+// network on its slice of perm in global memory (keys read through orig) and raises
+// MDRL_F_CELL_OVERFLOW to say so.
+constexpr int SORT_CAP = 2048, RANK_CAP = 256;
+__global__ __launch_bounds__(256) void k_cell_gather(const int *start, int *perm, const float4 *pos,
                                                      const float4 *vel, const int *type, const int *orig,
                                                      float4 *pos2, float4 *vel2, int *type2, int *orig2,
                                                      long long *fix, float *f32, int *slot_of,
@@ -1393,6 +1406,7 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *start, const int
     __shared__ int s_k[SORT_CAP];
     const int c = blockIdx.x;
     const int b = start[c], m = start[c + 1] - b;
+    const int nt = blockDim.x, tid = threadIdx.x;
     auto move = [&](int s, int o, int io) {
         pos2[s] = pos[o];
         vel2[s] = vel[o];
@@ -1402,27 +1416,60 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *start, const int
         if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
         if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     };
+    int np = 1;
+    while (np < m) np <<= 1;
     if (m > SORT_CAP) {
-        if (threadIdx.x == 0)
+        if (tid == 0)
             __hip_atomic_fetch_or(&counters[2], 4ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int k = threadIdx.x; k < m; k += blockDim.x) {
-            const int o = perm[b + k];
+        int *pv = perm + b;
+        for (int k = 2; k <= np; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = tid; t < (np >> 1); t += nt) {
+                    int lo, hi;
+                    bitonic_pair(t, k, j, lo, hi);
+                    if (hi >= m) continue;
+                    const int vl = pv[lo], vh = pv[hi];
+                    if (orig[vl] > orig[vh]) { pv[lo] = vh; pv[hi] = vl; }
+                }
+                __threadfence();   // the round's exchanges reach every thread of the workgroup
+                __syncthreads();
+            }
+        for (int k = tid; k < m; k += nt) {
+            const int o = pv[k];
             move(b + k, o, orig[o]);
         }
         return;
     }
-    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+    for (int k = tid; k < m; k += nt) {
         const int v = perm[b + k];
         s_v[k] = v;
         s_k[k] = orig[v];
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        const int key = s_k[k];
-        int r = 0;
-        for (int u = 0; u < m; ++u) r += (s_k[u] < key);
-        move(b + r, s_v[k], key);
+    if (m <= RANK_CAP) {
+        for (int k = tid; k < m; k += nt) {
+            const int key = s_k[k];
+            int r = 0;
+            for (int u = 0; u < m; ++u) r += (s_k[u] < key);
+            move(b + r, s_v[k], key);
+        }
+        return;
     }
+    for (int k = 2; k <= np; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < (np >> 1); t += nt) {
+                int lo, hi;
+                bitonic_pair(t, k, j, lo, hi);
+                if (hi >= m) continue;
+                const int kl = s_k[lo], kh = s_k[hi];
+                if (kl > kh) {
+                    s_k[lo] = kh; s_k[hi] = kl;
+                    const int v = s_v[lo]; s_v[lo] = s_v[hi]; s_v[hi] = v;
+                }
+            }
+            __syncthreads();
+        }
+    for (int k = tid; k < m; k += nt) move(b + k, s_v[k], s_k[k]);
 }
 
 int launch_bin_initial(DeviceSystem &s, hipStream_t st)
@@ -1439,7 +1486,10 @@ static TaskOrder task_order_args(const DeviceSystem &s)
     TaskOrder to{};
     to.geom = s.geom;
     to.cell_sys = s.cell_sys;
-    to.tasks = s.tasks;
+    // Only when the tasks fit the resident grid: with more tasks the work queues balance the
+    // load, while the pass would add ~75 us (1 M-atom system) to ~170 us (512 replicas) to
+    // every 16th binning scan; the host's list (cost per system, cells in order) is kept.
+    to.tasks = s.ncells * s.sched.groups <= 2 * s.cu_count ? s.tasks : nullptr;
     to.ngroups = s.sched.groups;
     group_weights(to.ngroups, to.w);
     // the pairing placement assumes the resident grid of the launch: range-limited tasks only
@@ -1449,8 +1499,9 @@ static TaskOrder task_order_args(const DeviceSystem &s)
 
 int launch_task_order(DeviceSystem &s, hipStream_t st)
 {
-    if (!s.tasks || s.ncells == 0) return MDRL_OK;
-    hipLaunchKernelGGL(k_task_order, dim3(1), dim3(1024), 0, st, s.ncells, s.cell_start, task_order_args(s));
+    const TaskOrder to = task_order_args(s);
+    if (!to.tasks || s.ncells == 0) return MDRL_OK;   // no device order for this schedule
+    hipLaunchKernelGGL(k_task_order, dim3(1), dim3(1024), 0, st, s.ncells, s.cell_start, to);
     MDRL_HIP(hipGetLastError());
     return MDRL_OK;
 }
