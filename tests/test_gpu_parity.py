@@ -445,9 +445,9 @@ def test_ragged_cells_bit_exact(apoa1, mode):
 
 def test_cell_over_sort_capacity(apoa1):
     """7,951 atoms held as one cell (a 43 A box with an explicit 1 x 1 x 1 grid, above the
-    2,048 atoms the in-cell sort holds): the gather keeps the scatter order and raises
-    MDRL_F_CELL_OVERFLOW, and the int64 forces, which do not depend on order, equal the
-    oracle's before and after two steps."""
+    2,048 atoms the in-cell sort holds in LDS): the gather sorts the cell in global memory
+    and raises MDRL_F_CELL_OVERFLOW, and the int64 forces equal the oracle's before and
+    after two steps (a lost or repeated atom in the gather would change them)."""
     s = mdrl.SyntheticSystem(4, 24)
     say("one-cell system: n=%d box %.1f" % (s.n, s.box[0]))
     g = mdrl.System(s.n, s.box, s.rc, s.eps, s.sigma, s.mass, dt=s.dt, coulomb_k=s.coulomb_k,
