@@ -189,49 +189,35 @@ __device__ __forceinline__ float pair_geom(float4 pi, float4 pj, float sx, float
     }
 }
 
-// Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3).  Entries [win_lo, win_lo + nwin)
-// -- the bins reachable below rc2 down to a floor seven octaves lower -- are read from
-// LDS; the rare r2 outside (overlapping atoms, or the reference's segment-0 fallback)
-// re-reads its entry from the global copy.  Inside the window the entry is
-// (bits(r2) >> 15) - off*256 - win_lo: exponent and mantissa[22:15] of r2 in one shift,
-// with no clamp (the window lies inside segments [0, nseg)).
-template <bool COUL, int STR>
-__device__ __forceinline__ void table_coeffs(float r2, const float *s_tab, const float *__restrict__ g_tab,
-                                             int off, int nseg, int win_lo, int nwin, float c[6])
+// Table coefficients of r2 (c0, c1 of r^-14, r^-8, r^-3) from the global table (6 floats
+// per entry), entry {exp8(r2) - offset, mantissa[22:15]} as the reference addresses it.
+// The reads go through the vector L1/L2 rather than an LDS copy: the force kernel's LDS is
+// busy with its tiles, queues and accumulator atomics, and random table gathers there cost
+// bank conflicts (92k system: 0.181 ms per force launch with a window of the table in LDS,
+// 0.160 ms with these reads).
+template <bool COUL>
+__device__ __forceinline__ void table_coeffs(float r2, const float *__restrict__ g_tab, int off, int nseg,
+                                             float c[6])
 {
-    const unsigned wi = (__float_as_uint(r2) >> 15) - (unsigned)(off * 256 + win_lo);
-    const bool inw = wi < (unsigned)nwin;
-    // outside the window: the last entry (in bounds; its values are replaced below)
-    const unsigned li = min(wi, (unsigned)max(nwin - 1, 0));
-    if (STR == 4) {
-        const float4 v = *reinterpret_cast<const float4 *>(s_tab + (li << 2));
-        c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w; c[4] = 0.f; c[5] = 0.f;
+    const int idx = lut_seg(r2, off, nseg) * 256 + lut_bin(r2);
+    const float *q = g_tab + (size_t)idx * 6;
+    const float2 u = *reinterpret_cast<const float2 *>(q);
+    const float2 v = *reinterpret_cast<const float2 *>(q + 2);
+    c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y;
+    if (COUL) {
+        const float2 w = *reinterpret_cast<const float2 *>(q + 4);
+        c[4] = w.x; c[5] = w.y;
     } else {
-        // 24-bit multiply-add of the byte offset: one full-rate instruction
-        const float *e = reinterpret_cast<const float *>(reinterpret_cast<const char *>(s_tab) + __umul24(li, 24u));
-        const float2 u = *reinterpret_cast<const float2 *>(e);
-        const float2 v = *reinterpret_cast<const float2 *>(e + 2);
-        const float2 w = *reinterpret_cast<const float2 *>(e + 4);
-        c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y; c[4] = w.x; c[5] = w.y;
-    }
-    if (!inw) {
-        const int idx = lut_seg(r2, off, nseg) * 256 + lut_bin(r2);
-        const float *q = g_tab + (size_t)idx * 6;
-        c[0] = q[0]; c[1] = q[1]; c[2] = q[2]; c[3] = q[3];
-        if (COUL) { c[4] = q[4]; c[5] = q[5]; }
-        // complete these loads here, so that the common path does not wait on the
-        // caller's other global traffic (prefetches, flush atomics) in flight
-        __builtin_amdgcn_s_waitcnt(0);
+        c[4] = 0.f; c[5] = 0.f;
     }
 }
 
 // Scalar pair force F (kJ/mol/nm per A of separation): table path
 // D/RL_LJ_Evaluate_Pairs_1st_Order.v:409-475 with the per-pair multipliers of
 // D/RL_Evaluate_Pairs_1st_Order.v:53-55 (A = 10 eps sig^12, B = 10 eps sig^6, qq = qs_i qs_j).
-template <bool COUL, bool PAR, bool ANA, int STR>
+template <bool COUL, bool PAR, bool ANA>
 __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float qq,
-                                             const float *s_tab, const float *__restrict__ g_tab,
-                                             int off, int nseg, int win_lo, int nwin)
+                                             const float *__restrict__ g_tab, int off, int nseg)
 {
     if (ANA) {
         float ir2 = 1.0f / r2;
@@ -248,7 +234,7 @@ __device__ __forceinline__ float pair_scalar(float r2, float A, float B, float q
         return F;
     }
     float c[6];
-    table_coeffs<COUL, STR>(r2, s_tab, g_tab, off, nseg, win_lo, nwin, c);
+    table_coeffs<COUL>(r2, g_tab, off, nseg, c);
     float F;
     if (PAR) {
         float t14 = c[1] * r2; t14 = t14 + c[0];

@@ -377,8 +377,8 @@ template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = f
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_rl_halfshell(ForceArgs a)
 {
     static_assert(!EN || (!F32 && !PAR && !ANA && APW == 64), "the energy pass uses the table path");
-    constexpr int FSTR = COUL ? 6 : 4;   // coefficients per table entry (force, or EN energy)
-    constexpr int STR = FSTR;            // floats per window entry
+    constexpr int FSTR = COUL ? 6 : 4;   // coefficients per table entry
+    constexpr int STR = FSTR;            // floats per entry of the energy pass's window
     constexpr int NACC = EN == 1 ? 8 : 2;   // EN: E, E(rc)[, W xx xy xz yy yz zz]
     constexpr int P = 64 / APW;
     static_assert(TJ <= 512 && TJ <= NTHR, "queue entries hold 9-bit neighbour indices");
@@ -411,10 +411,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
     const int il = lane % APW, ph = lane / APW;
 
-    // table window and type-pair multipliers
-    if (!ANA) {
+    // the energy pass's window of the energy tables (the force path reads its table from
+    // global memory), and the type-pair multipliers
+    if (EN) {
         for (int e = tid; e < a.nwin; e += NTHR) {
-            const float *p = (EN ? a.etab : a.tab) + (size_t)(a.win_lo + e) * 6;
+            const float *p = a.etab + (size_t)(a.win_lo + e) * 6;
             float *q = s_tab + e * STR;
             q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[3];
             if (FSTR == 6) { q[4] = p[4]; q[5] = p[5]; }
@@ -729,8 +730,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                             Ue = fmaf(qq, a.u1c, Ue);
                         }
                     } else {
-                        F = pair_scalar<COUL, PAR, ANA, STR>(r2, ab.x, ab.y, qq, s_tab, a.tab,
-                                                             a.off, a.nseg, a.win_lo, a.nwin);
+                        F = pair_scalar<COUL, PAR, ANA>(r2, ab.x, ab.y, qq, a.tab, a.off, a.nseg);
                     }
                     // the reference filter 0 < r2 <= rc2 on the pair's own r2 (the scan passed
                     // a superset): one unsigned compare of the bit patterns (r2 >= 0)
@@ -1041,9 +1041,10 @@ template <bool COUL, bool F32, bool PAR, bool ANA, int APW, int TJ, bool EXC = f
 static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st)
 {
     // Two workgroups share a CU's 160 KiB of LDS only while each stays within 80 KiB: the
-    // table window keeps its top (the bins just below rc^2, where most pairs fall) and is cut
-    // from below to what fits beside this variant's static arrays; entries below it are read
-    // from the global table, which gives the same bits.
+    // energy pass's table window keeps its top (the bins just below rc^2, where most pairs
+    // fall) and is cut from below to what fits beside this variant's static arrays; entries
+    // below it are read from the global table, which gives the same bits.  (The force path
+    // has no window: lds = 0.)
     static size_t static_lds = ~(size_t)0;
     if (static_lds == ~(size_t)0) {
         hipFuncAttributes fa{};
@@ -1105,15 +1106,14 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.rc2_lo = s.rc2_lo;
     a.rc2_64 = s.rc2_64;
     a.tab = s.tab; a.off = s.exp_offset; a.nseg = s.nseg;
-    a.win_lo = s.win.lo; a.nwin = s.win.n;
+    a.win_lo = 0; a.nwin = 0;   // the force path reads its table from global memory
     a.pair_ab = s.pair_ab; a.ntypes = s.ntypes;
     a.fix = s.fix; a.f32 = s.f32; a.counters = s.counters;
     a.ngroups = s.sched.groups;
     offset_groups(a.ngroups, a.gstart);
     a.ntasks = s.ncells * a.ngroups;
     a.orig = s.orig; a.excl = s.excl;
-    const int str = coul ? 6 : 4;
-    const size_t lds = ana ? 0 : (size_t)s.win.n * str * sizeof(float);
+    const size_t lds = 0;
     const int ntot = a.ntasks + a.bd.nblocks;
     const int grid_n = s.sched.persistent ? std::min(ntot, 2 * s.cu_count) : ntot;
     a.progress_prio = ntot <= 2 * grid_n;
