@@ -585,7 +585,8 @@ def main():
     # beforehand; timed region: the steps as a user runs them (graph replays unless --graph 0)
     step_s = run_timed(g, args.warmup, args.steps, bool(args.graph))
     wall = step_s * args.steps
-    pairs_last = g.stats().pair_evals
+    st_last = g.stats()
+    pairs_last = st_last.pair_evals
     fix_end, pos_end = g.forces_fixed(), g.particles()
     # kernel time: the same number of steps again, each force launch bracketed by HIP
     # events recorded on the stream it runs on
@@ -642,6 +643,9 @@ def main():
         },
         "ns_per_day": ns_day,
         "force_kernel_ms": st.force_ms,
+        # candidate r2 tests of the last timed step per in-cutoff pair (whole 64-lane steps,
+        # idle lanes of partly filled chunks included)
+        "candidates_per_pair_eval": st_last.candidates / max(st_last.pair_evals, 1),
         "schedule": dict(sched.as_dict(), autotune_best_ms=tune_ms, autotune_cache_hit=tune_hit),
         "dt_fs": s.dt,
         "step_device_ms_eager": st.step_ms,
