@@ -283,6 +283,19 @@ __device__ __forceinline__ void bonded_block(const BondedArgs &a, int4 blk, unsi
     }
 }
 
+// Minimum over the wave: swizzles within each half (no address registers), then the two
+// halves.  Exact in any order.
+__device__ __forceinline__ float wave_min(float v)
+{
+    v = fminf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (1 << 10) | 0x1f)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (2 << 10) | 0x1f)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (4 << 10) | 0x1f)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (8 << 10) | 0x1f)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (16 << 10) | 0x1f)));
+    return fminf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
+}
+
 // ---------------------------------------------------------------------------------
 // k_rl_halfshell: one workgroup of 12 waves per task = (home cell, group of half-shell
 // offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
@@ -432,7 +445,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     float *s_jacc_f = reinterpret_cast<float *>(s_jacc);
     unsigned *q = s_q[wave];
     unsigned flag = 0;
-    unsigned long long npairs = 0, ncand = 0;   // pairs evaluated and kept; candidate tests
+    // pairs evaluated and kept, and candidate tests of 64 lanes, by this wave (far below
+    // 2^32: a launch's own totals are)
+    unsigned npairs = 0, ncand = 0;
     const float rc2s = a.rc2_scan;
 
     // One block of bonded terms: forces accumulate in the home-tile accumulators
@@ -484,9 +499,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
         const int hz = hl % nc2, hy = (hl / nc2) % nc1, hx = hl / (nc1 * nc2);
         const int hstart = a.cell_start[h], hcount = a.cell_start[h + 1] - hstart;
         const int obeg = a.gstart[grp], oend = a.gstart[grp + 1];
-        // the home cell's corner: origin of the scan's relative coordinates
-        const float x0 = (float)hx * (G.box[0] / (float)nc0), y0 = (float)hy * (G.box[1] / (float)nc1),
-                    z0 = (float)hz * (G.box[2] / (float)nc2);
+        // the home cell's corner: origin of the scan's relative coordinates (the same on
+        // every lane: held in scalar registers)
+        auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+        const float x0 = uni((float)hx * (G.box[0] / (float)nc0)), y0 = uni((float)hy * (G.box[1] / (float)nc1)),
+                    z0 = uni((float)hz * (G.box[2] / (float)nc2));
         // the scan's record of a staged neighbour: image relative to the corner, queue bits
         auto scan_rec = [&](float4 p, float4 sh, unsigned bits) {
             return make_float4((p.x + sh.x) - x0, (p.y + sh.y) - y0, (p.z + sh.z) - z0, __uint_as_float(bits));
@@ -574,12 +591,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 if (EXC) of = a.orig[sl];
             }
             if (tid < TI) {
-#pragma unroll
-                for (int m = 32; m > 0; m >>= 1) {
-                    blx = fminf(blx, __shfl_xor(blx, m)); bly = fminf(bly, __shfl_xor(bly, m));
-                    blz = fminf(blz, __shfl_xor(blz, m)); bhx = fmaxf(bhx, __shfl_xor(bhx, m));
-                    bhy = fmaxf(bhy, __shfl_xor(bhy, m)); bhz = fmaxf(bhz, __shfl_xor(bhz, m));
-                }
+                blx = wave_min(blx); bly = wave_min(bly); blz = wave_min(blz);
+                bhx = -wave_min(-bhx); bhy = -wave_min(-bhy); bhz = -wave_min(-bhz);
                 if (lane == 0) {
                     s_hb[wave][0] = blx; s_hb[wave][1] = bly; s_hb[wave][2] = blz;
                     s_hb[wave][3] = bhx; s_hb[wave][4] = bhy; s_hb[wave][5] = bhz;
@@ -595,7 +608,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 bhx = fmaxf(bhx, s_hb[w][3]); bhy = fmaxf(bhy, s_hb[w][4]); bhz = fmaxf(bhz, s_hb[w][5]);
             }
             // (the box is the same on every lane: held in scalar registers)
-            auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
             blx = uni(blx); bly = uni(bly); blz = uni(blz);
             bhx = uni(bhx); bhy = uni(bhy); bhz = uni(bhz);
             // A neighbour image farther than rc from the home tile's box has no pair with
@@ -618,6 +630,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
             float4 ksh = make_float4(0.f, 0.f, 0.f, 0.f);
             auto keep_test = [&](const NbrTile &t, float4 p, int par) {
                 bool keep = false;
+                ksl = 0;
+                ksh = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (tid < t.mj) {
                     ksl = locate(t, tid, ksh);
                     keep = (t.o == 0) || near_box(scan_rec(p, ksh, 0u));
@@ -681,7 +695,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 // queue entry: home index + neighbour index << 8 + type pair << 17.  The scan
                 // queues r2 <= rc2; a pair at r2 == 0 (coincident atoms, outside the
                 // reference's 0 < r2 filter) or an excluded pair is dropped here, uncounted.
-                auto eval = [&](unsigned e) {
+                auto eval = [&](unsigned e) -> bool {
                     const int ia = (int)(e & 0xffu), jb = (int)((e >> kJbShift) & kJbMask);
                     const float4 pa = s_i[ia], pb = s_j[jb];
                     float dx, dy, dz;
@@ -752,7 +766,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         drop = drop || ((unsigned)d < 32u && (((unsigned)om.y >> d) & 1u));
                     }
                     const bool keep = !drop;
-                    npairs += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(keep));
                     F = keep ? F : 0.0f;
                     if (EN) {
                         // the pair's energy, its energy at rc and its virial F d (x) d
@@ -801,6 +814,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         atomic_sub_wg(&s_jacc[3 * jb + 1], (unsigned long long)ky);
                         atomic_sub_wg(&s_jacc[3 * jb + 2], (unsigned long long)kz);
                     }
+                    return keep;
                 };
 
                 // the wave's queue holds entries q[0, tail), tail < 64 between tests
@@ -859,7 +873,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                                     if (tail >= 64u) {
                                         // evaluate the first 64, move the rest (< 64) to the front
                                         __builtin_amdgcn_wave_barrier();
-                                        eval(q[lane]);
+                                        npairs += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(eval(q[lane])));
                                         const unsigned rest = tail - 64u;
                                         const unsigned mv = q[64 + lane];
                                         __builtin_amdgcn_wave_barrier();
@@ -873,11 +887,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     };
                     if (self) run_steps(std::true_type{});
                     else run_steps(std::false_type{});
-                    ncand += (unsigned long long)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned long long)nch;
+                    ncand += (unsigned)((nsteps - wave + NWAVE - 1) / NWAVE) * (unsigned)nch;
                 }
                 if (tail) {
                     __builtin_amdgcn_wave_barrier();
-                    if ((unsigned)lane < tail) eval(q[lane]);
+                    bool kp = false;
+                    if ((unsigned)lane < tail) kp = eval(q[lane]);
+                    npairs += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(kp));
                     __builtin_amdgcn_wave_barrier();
                 }
                 keep_test(nxt, pf, jpar ^ 1);
@@ -959,7 +975,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     // statistics (npairs/ncand are wave-uniform): one add per workgroup into its slot
     if (lane == 0) {
         s_stat[wave][0] = npairs;
-        s_stat[wave][1] = ncand * 64ull;
+        s_stat[wave][1] = (unsigned long long)ncand * 64ull;
     }
     lds_barrier();
     if (tid < 2) {
@@ -1389,25 +1405,38 @@ __device__ __forceinline__ void bitonic_pair(int t, int k, int j, int &lo, int &
     else { lo = blk * 2 * j + off; hi = lo + j; }
 }
 
-// One workgroup per cell: orders the cell's slots by input index and moves their particle
-// data into the new slots, clearing the forces and recording each input index's slot.
-// Up to 256 atoms each thread ranks its atom against the cell in LDS; up to SORT_CAP a
-// bitonic network sorts (input index, slot) pairs in LDS; a larger cell runs the same
-// network on its slice of perm in global memory (keys read through orig) and raises
-// MDRL_F_CELL_OVERFLOW to say so.
+// One workgroup per cell: orders the cell's slots along the cell's longest edge (the
+// coordinate's bit pattern, ties by input index: a total order fixed by the positions alone)
+// and moves their particle data into the new slots, clearing the forces and recording each
+// input index's slot.  The force kernel's lanes then hold home chunks of 64 atoms that are
+// slabs across that edge.  Up to 256 atoms each thread ranks its atom against the
+// cell in LDS; up to SORT_CAP a bitonic network sorts (key, slot) pairs in LDS; a larger
+// cell runs the same network on its slice of perm in global memory (keys read through pos
+// and orig) and raises MDRL_F_CELL_OVERFLOW to say so.
 constexpr int SORT_CAP = 2048, RANK_CAP = 256;
-__global__ __launch_bounds__(256) void k_cell_gather(const int *start, int *perm, const float4 *pos,
+__global__ __launch_bounds__(RANK_CAP) void k_cell_gather(const int *start, int *perm, const float4 *pos,
                                                      const float4 *vel, const int *type, const int *orig,
                                                      float4 *pos2, float4 *vel2, int *type2, int *orig2,
                                                      long long *fix, float *f32, int *slot_of,
-                                                     unsigned long long *counters)
+                                                     unsigned long long *counters, const SysGeom *geom,
+                                                     const int *cell_sys)
 {
     __shared__ int s_v[SORT_CAP];
-    __shared__ int s_k[SORT_CAP];
+    __shared__ long long s_k[SORT_CAP];
     const int c = blockIdx.x;
     const int b = start[c], m = start[c + 1] - b;
     const int nt = blockDim.x, tid = threadIdx.x;
-    auto move = [&](int s, int o, int io) {
+    // the sort axis: the cell's longest edge (the first of equal ones)
+    const SysGeom &G = geom[cell_sys[c]];
+    const float ex = G.box[0] / (float)G.nc[0], ey = G.box[1] / (float)G.nc[1], ez = G.box[2] / (float)G.nc[2];
+    const int ax = (ez > ex && ez > ey) ? 2 : (ey > ex ? 1 : 0);
+    auto key_at = [&](float4 p, int io) {
+        const float x = ax == 0 ? p.x : ax == 1 ? p.y : p.z;
+        return (long long)(((unsigned long long)(unsigned)__float_as_int(x) << 32) | (unsigned)io);
+    };
+    auto key_of = [&](int v) { return key_at(pos[v], orig[v]); };
+    auto move = [&](int s, int o) {
+        const int io = orig[o];
         pos2[s] = pos[o];
         vel2[s] = vel[o];
         type2[s] = type[o];
@@ -1416,6 +1445,36 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *start, int *perm
         if (fix) { fix[3 * (size_t)s] = 0; fix[3 * (size_t)s + 1] = 0; fix[3 * (size_t)s + 2] = 0; }
         if (f32) reinterpret_cast<float4 *>(f32)[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     };
+    if (m <= RANK_CAP) {
+        // one atom per thread (RANK_CAP is the workgroup size): its data stays in registers
+        // while it ranks its key against the cell's, then goes to its slot
+        int io = 0, ty = 0;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f), vv = p;
+        long long key = 0;
+        if (tid < m) {
+            const int v = perm[b + tid];
+            p = pos[v];
+            vv = vel[v];
+            ty = type[v];
+            io = orig[v];
+            key = key_at(p, io);
+            s_k[tid] = key;
+        }
+        __syncthreads();
+        if (tid < m) {
+            int r = 0;
+            for (int u = 0; u < m; ++u) r += (s_k[u] < key);
+            const int sl = b + r;
+            pos2[sl] = p;
+            vel2[sl] = vv;
+            type2[sl] = ty;
+            orig2[sl] = io;
+            if (slot_of) slot_of[io] = sl;
+            if (fix) { fix[3 * (size_t)sl] = 0; fix[3 * (size_t)sl + 1] = 0; fix[3 * (size_t)sl + 2] = 0; }
+            if (f32) reinterpret_cast<float4 *>(f32)[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        return;
+    }
     int np = 1;
     while (np < m) np <<= 1;
     if (m > SORT_CAP) {
@@ -1429,39 +1488,27 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *start, int *perm
                     bitonic_pair(t, k, j, lo, hi);
                     if (hi >= m) continue;
                     const int vl = pv[lo], vh = pv[hi];
-                    if (orig[vl] > orig[vh]) { pv[lo] = vh; pv[hi] = vl; }
+                    if (key_of(vl) > key_of(vh)) { pv[lo] = vh; pv[hi] = vl; }
                 }
                 __threadfence();   // the round's exchanges reach every thread of the workgroup
                 __syncthreads();
             }
-        for (int k = tid; k < m; k += nt) {
-            const int o = pv[k];
-            move(b + k, o, orig[o]);
-        }
+        for (int k = tid; k < m; k += nt) move(b + k, pv[k]);
         return;
     }
     for (int k = tid; k < m; k += nt) {
         const int v = perm[b + k];
         s_v[k] = v;
-        s_k[k] = orig[v];
+        s_k[k] = key_of(v);
     }
     __syncthreads();
-    if (m <= RANK_CAP) {
-        for (int k = tid; k < m; k += nt) {
-            const int key = s_k[k];
-            int r = 0;
-            for (int u = 0; u < m; ++u) r += (s_k[u] < key);
-            move(b + r, s_v[k], key);
-        }
-        return;
-    }
     for (int k = 2; k <= np; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = tid; t < (np >> 1); t += nt) {
                 int lo, hi;
                 bitonic_pair(t, k, j, lo, hi);
                 if (hi >= m) continue;
-                const int kl = s_k[lo], kh = s_k[hi];
+                const long long kl = s_k[lo], kh = s_k[hi];
                 if (kl > kh) {
                     s_k[lo] = kh; s_k[hi] = kl;
                     const int v = s_v[lo]; s_v[lo] = s_v[hi]; s_v[hi] = v;
@@ -1469,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *start, int *perm
             }
             __syncthreads();
         }
-    for (int k = tid; k < m; k += nt) move(b + k, s_v[k], s_k[k]);
+    for (int k = tid; k < m; k += nt) move(b + k, s_v[k]);
 }
 
 int launch_bin_initial(DeviceSystem &s, hipStream_t st)
@@ -1513,10 +1560,11 @@ int launch_rebin(DeviceSystem &s, hipStream_t st)
     if (s.n > 0) {
         hipLaunchKernelGGL(k_scatter, dim3(nblk(s.n, 256)), dim3(256), 0, st, s.n, s.cell,
                            s.cell_fill, s.perm, s.ncells);
-        hipLaunchKernelGGL(k_cell_gather, dim3(s.ncells), dim3(256), 0, st, s.cell_start, s.perm,
+        hipLaunchKernelGGL(k_cell_gather, dim3(s.ncells), dim3(RANK_CAP), 0, st, s.cell_start, s.perm,
                            s.pos, s.vel, s.type, s.orig, s.pos2, s.vel2, s.type2, s.orig2,
                            (s.flags & MDRL_ACC_FP32) ? nullptr : s.fix,
-                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr, s.slot_of, s.counters);
+                           (s.flags & MDRL_ACC_FP32) ? s.f32 : nullptr, s.slot_of, s.counters,
+                           s.geom, s.cell_sys);
     }
     MDRL_HIP(hipGetLastError());
     std::swap(s.pos, s.pos2);
