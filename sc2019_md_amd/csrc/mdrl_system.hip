@@ -296,6 +296,31 @@ __device__ __forceinline__ float wave_min(float v)
                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
 }
 
+// Maximum over the wave of an int, the same way.
+__device__ __forceinline__ int wave_max(int v)
+{
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, (1 << 10) | 0x1f));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, (2 << 10) | 0x1f));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1f));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, (8 << 10) | 0x1f));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1f));
+    return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+}
+
+// Inclusive prefix sum over the wave: within each row of 16 lanes by DPP row shifts (lanes
+// shifted in from outside the row contribute 0), then the totals of the rows before.
+__device__ __forceinline__ int wave_prefix(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31),
+              r2 = __builtin_amdgcn_readlane(x, 47);
+    const int row = (int)(threadIdx.x & 63) >> 4;
+    return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+}
+
 // ---------------------------------------------------------------------------------
 // k_rl_halfshell: one workgroup of 12 waves per task = (home cell, group of half-shell
 // offsets), or, in persistent mode, a fixed grid of workgroups pulling tasks (largest
@@ -1278,12 +1303,7 @@ __device__ __forceinline__ void order_tasks(int ncells, const int *start, const 
             // exclusive scan of the bin counts: first position of each bin
             const int lane = t & 63, w = t >> 6;
             const int v = s_sum[t];
-            int inc = v;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_up(inc, off);
-                if (lane >= off) inc += u;
-            }
+            const int inc = wave_prefix(v);
             if (lane == 63) s_wsum[w] = inc;
             __syncthreads();
             int before = 0;
@@ -1336,15 +1356,9 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
         }
         mx = max(mx, tot);
         if (c < ncells && c < kOrderCells) s_cnt[c] = tot;
-        // inclusive scan of the tile: within each wave by shuffles, then across the 16
-        // wave totals
+        // inclusive scan of the tile: within each wave, then across the 16 wave totals
         const int lane = t & 63, w = t >> 6;
-        int inc = tot;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int u = __shfl_up(inc, off);
-            if (lane >= off) inc += u;
-        }
+        const int inc = wave_prefix(tot);
         if (lane == 63) s_wsum[w] = inc;
         __syncthreads();
         int before = s_run;
@@ -1364,16 +1378,14 @@ __global__ __launch_bounds__(1024) void k_scan(int ncells, int *count, int *star
         if (t == 1023) s_run = before + inc;
         __syncthreads();
     }
-    // largest cell
-    s_sum[t] = mx;
+    // largest cell: per wave, then over the 16 waves
+    mx = wave_max(mx);
+    if ((t & 63) == 0) s_sum[t >> 6] = mx;
     __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
-        if (t < off) s_sum[t] = max(s_sum[t], s_sum[t + off]);
-        __syncthreads();
-    }
     if (t == 0) {
+        for (int k = 1; k < 16; ++k) mx = max(mx, s_sum[k]);
         start[ncells] = s_run;
-        counters[3] = (unsigned long long)s_sum[0];
+        counters[3] = (unsigned long long)mx;
     }
     if (t < 2 * kStatSlots + kQueues) counters[8 + t] = 0ull;
     if (!to.tasks) return;
