@@ -19,14 +19,15 @@ namespace mdrl {
 // ---------------------------------------------------------------------------------
 
 // Table entry of r2: {exp8(r2) - offset, mantissa[22:15]}, segment 0 when out of range.
-// D/RL_LJ_Evaluate_Pairs_1st_Order.v:176-199.
-__device__ __forceinline__ int lut_seg(float r2, int off, int nseg)
+// D/RL_LJ_Evaluate_Pairs_1st_Order.v:176-199.  Bits 30..15 of r2 are exp8 << 8 | bin, so
+// the entry is those bits less offset << 8 when that lies below nseg << 8 (one unsigned
+// compare covers both ends), else the bin alone.
+__device__ __forceinline__ unsigned lut_index(float r2, int off, int nseg)
 {
-    int e = (int)((__float_as_uint(r2) >> 23) & 0xffu);
-    int seg = e - off;
-    return (seg < 0 || seg >= nseg) ? 0 : seg;
+    const unsigned raw = (__float_as_uint(r2) >> 15) & 0xffffu;
+    const unsigned rel = raw - ((unsigned)off << 8);
+    return rel < ((unsigned)nseg << 8) ? rel : (raw & 0xffu);
 }
-__device__ __forceinline__ int lut_bin(float r2) { return (int)((__float_as_uint(r2) >> 15) & 0xffu); }
 
 // rint(f * 2^32) as int64, saturated at +-2^62 (flag bit 0 set on saturation or NaN).
 // Fast path: for |f*2^32| < 2^51 the sum f*2^32 + 1.5*2^52 (one rounding, the fma) is an
@@ -199,8 +200,9 @@ template <bool COUL>
 __device__ __forceinline__ void table_coeffs(float r2, const float *__restrict__ g_tab, int off, int nseg,
                                              float c[6])
 {
-    const int idx = lut_seg(r2, off, nseg) * 256 + lut_bin(r2);
-    const float *q = g_tab + (size_t)idx * 6;
+    // (a 32-bit byte offset from the table's base, so the loads take the scalar-base form)
+    const float *q = reinterpret_cast<const float *>(reinterpret_cast<const char *>(g_tab) +
+                                                     __umul24(lut_index(r2, off, nseg), 24u));
     const float2 u = *reinterpret_cast<const float2 *>(q);
     const float2 v = *reinterpret_cast<const float2 *>(q + 2);
     c[0] = u.x; c[1] = u.y; c[2] = v.x; c[3] = v.y;

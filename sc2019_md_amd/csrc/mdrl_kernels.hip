@@ -36,8 +36,7 @@ __global__ void k_eval_pairs(const float *tab, int off, int nseg, const float *r
     bool in = (r2 > 0.0f) && (strict ? (r2 < rc2) : (r2 <= rc2));
     float F = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
     if (in) {
-        const int idx = lut_seg(r2, off, nseg) * 256 + lut_bin(r2);
-        const float *c = tab + (size_t)idx * 6;
+        const float *c = tab + (size_t)lut_index(r2, off, nseg) * 6;
         const float A = pa ? pa[p] : 1.0f, B = pb ? pb[p] : 1.0f, Q = pqq ? pqq[p] : 0.0f;
         if (par) {
             float t14 = c[1] * r2; t14 = t14 + c[0];

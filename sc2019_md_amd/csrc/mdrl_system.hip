@@ -746,7 +746,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                         // the caches hold (a window of both would halve the entries in LDS)
                         float c[6] = {0, 0, 0, 0, 0, 0}, ce[6] = {0, 0, 0, 0, 0, 0};
                         const unsigned wi = (__float_as_uint(r2) >> 15) - (unsigned)(a.off * 256 + a.win_lo);
-                        const size_t idx = (size_t)(lut_seg(r2, a.off, a.nseg) * 256 + lut_bin(r2)) * 6;
+                        const size_t idx = (size_t)lut_index(r2, a.off, a.nseg) * 6;
                         if (EN == 1) {
 #pragma unroll
                             for (int k = 0; k < FSTR; ++k) c[k] = a.tab[idx + k];
