@@ -349,7 +349,9 @@ __device__ __forceinline__ int wave_prefix(int x)
 
 // 12 waves per workgroup: two workgroups (24 waves) fill a CU's LDS and its VGPRs at
 // 6 waves per SIMD.
-constexpr int TI = 256, QCAP = 128, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
+constexpr int TI = 256, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
+// per-wave pair queue: fewer than 64 entries between steps, up to 64 more per chunk of a step
+constexpr int QCAP = 64 + 64 * NCHUNK;
 // queue entry: home index (8 bits) | neighbour index (9 bits) << 8 | type pair << 17
 constexpr unsigned kJbShift = 8, kJbMask = 0x1ffu, kTpShift = 17;
 static_assert(TI <= 256, "queue entries hold 8-bit home indices");
@@ -842,7 +844,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                     return keep;
                 };
 
-                // the wave's queue holds entries q[0, tail), tail < 64 between tests
+                // the wave's queue holds entries q[0, tail), tail < 64 between steps
                 unsigned tail = 0;
                 const int bj = mj > 0 ? il % mj : 0;
                 const int nsteps = (mj + P - 1) / P;
@@ -895,18 +897,24 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                                         qt[__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = eb[ch] + jsh;
                                     tail += (unsigned)__popcll(m);
-                                    if (tail >= 64u) {
-                                        // evaluate the first 64, move the rest (< 64) to the front
-                                        __builtin_amdgcn_wave_barrier();
-                                        npairs += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(eval(q[lane])));
-                                        const unsigned rest = tail - 64u;
-                                        const unsigned mv = q[64 + lane];
-                                        __builtin_amdgcn_wave_barrier();
-                                        if ((unsigned)lane < rest) q[lane] = mv;
-                                        __builtin_amdgcn_wave_barrier();
-                                        tail = rest;
-                                    }
                                 }
+                            }
+                            if (tail >= 64u) {
+                                // evaluate the queue's full batches of 64, then move the rest
+                                // (< 64) to the front
+                                unsigned head = 0;
+                                do {
+                                    __builtin_amdgcn_wave_barrier();
+                                    npairs += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(eval(q[head + lane])));
+                                    head += 64u;
+                                } while (tail - head >= 64u);
+                                const unsigned rest = tail - head;
+                                unsigned mv = 0;
+                                if ((unsigned)lane < rest) mv = q[head + lane];
+                                __builtin_amdgcn_wave_barrier();
+                                if ((unsigned)lane < rest) q[lane] = mv;
+                                __builtin_amdgcn_wave_barrier();
+                                tail = rest;
                             }
                         }
                     };
