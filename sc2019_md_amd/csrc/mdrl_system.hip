@@ -338,8 +338,8 @@ __device__ __forceinline__ int wave_prefix(int x)
 // a step both indices of the queued pairs are distinct and one LDS read feeds every chunk;
 // steps are strided over the 12 waves.  The r2 <= rc2 test of D/Filter_Logic.v:234 is a
 // ballot; passing pairs are compacted into a per-wave LDS queue (the filter buffers and
-// arbiter of D/Filter_Bank.v:548-606) and, whenever 64 are queued, all 64 lanes evaluate
-// them densely: table force from an LDS window, then the reference particle gets +F and the
+// arbiter of D/Filter_Bank.v:548-606) and, after each step, all 64 lanes evaluate each full
+// batch of 64 densely: table force from the global table, then the reference particle gets +F and the
 // neighbour -F (D/RL_LJ_Evaluation_Unit.v:137) in LDS accumulators, int64 fixed point
 // (exact, order-free) or FP32.  Neighbour forces are flushed to HBM with one atomic per
 // touched particle and component after each neighbour tile
