@@ -256,8 +256,8 @@ def state_digest(ctxs_and_sys):
     return h.hexdigest()
 
 
-def plain_contexts(systems, table, relax=0, relax_step=0.02, tune=False):
-    plain = [mdrl.System.from_synthetic(s, table=table) for s in systems]
+def plain_contexts(systems, table, relax=0, relax_step=0.02, tune=False, flags=0):
+    plain = [mdrl.System.from_synthetic(s, flags=flags, table=table) for s in systems]
     for g in plain:
         if relax:
             g.minimize(relax, relax_step, 1e-4)
@@ -422,7 +422,9 @@ def bench_batch(args, which):
         for s in systems:
             s.dt = float(args.dt)
     s0 = systems[0]
-    b = mdrl.Batch.from_synthetic(systems)
+    coarse = args.coarse_cells if args.coarse_cells >= 0 else int(which == 5)
+    cflags = mdrl.COARSE_CELLS if coarse else 0
+    b = mdrl.Batch.from_synthetic(systems, flags=cflags)
     relax = args.relax if args.relax else 0
     if relax:
         b.minimize(relax, args.relax_step, 1e-4)
@@ -451,7 +453,7 @@ def bench_batch(args, which):
     nplain = min(args.replica_steps, args.steps)
     versus = None
     if nplain > 0:
-        b2 = mdrl.Batch.from_synthetic(systems)
+        b2 = mdrl.Batch.from_synthetic(systems, flags=cflags)
         if relax:
             b2.minimize(relax, args.relax_step, 1e-4)
         b2.set_schedule(sched.atoms_per_wave, sched.tile_depth, sched.offset_groups, sched.persistent)
@@ -462,7 +464,7 @@ def bench_batch(args, which):
         versus = {"steps": nplain, "warmup": args.warmup,
                   "batched": {"ms_per_step": tb2 * 1e3, "pair_evals_per_s": pairs_b2 / tb2}}
         for name, tune, graph in (("plain", True, True), ("plain_eager", False, False)):
-            plain = plain_contexts(systems, None, relax, args.relax_step, tune)
+            plain = plain_contexts(systems, None, relax, args.relax_step, tune, cflags)
             tp = time_plain(plain, args.warmup, nplain, graph)
             pairs_p = sum(int(g.stats().pair_evals) for g in plain)
             digests[name] = state_digest(plain)
@@ -503,6 +505,8 @@ def bench_batch(args, which):
             "pair_eval": "first-order tables, ApoA1 layout",
             "integrator": "velocity Verlet",
             "parallelism": "single GPU, one batch",
+            "cells": ("coarse: floor(L/rc) per axis, reduced while cells average < 128 atoms "
+                      "(MDRL_COARSE_CELLS)" if coarse else "floor(L/rc) per axis"),
         },
         "ns_per_day": (1e3 / ms) * s0.dt * 0.0864,
         "force_kernel_ms": st.force_ms,
@@ -544,6 +548,9 @@ def main():
                     help="also run one device copy of this many bytes (traffic counter calibration)")
     ap.add_argument("--schedule", default=None,
                     help="atoms_per_wave,tile_depth,offset_groups,persistent instead of the autotuner's choice")
+    ap.add_argument("--coarse-cells", type=int, default=-1,
+                    help="batched workloads: 1 = MDRL_COARSE_CELLS grid, 0 = floor(L/rc) grid, "
+                         "-1 = coarse for the bonded workload (sparse: 48 beads per rc-sized cell)")
     ap.add_argument("--long-range-field", action="store_true",
                     help="attach a summation-hook provider (a uniform field on the charges, copied "
                          "from a device buffer each evaluation) for the steps, to time the hook")

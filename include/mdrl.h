@@ -158,13 +158,20 @@ enum {
     MDRL_EVAL_PARITY = 4,    /* unfused arithmetic, d = (xi - xj) - image (reference order) */
     MDRL_KICK_DRIFT = 8,     /* reference integrator: v += a*dt; x += v*dt (no half kicks) */
     MDRL_NO_COULOMB = 16,    /* drop the r^-3 term even if charges are set */
+    MDRL_COARSE_CELLS = 32,  /* default grid coarsened while cells average < 128 atoms: faster
+                                for sparse systems; the pair set is the same, but a pair across
+                                the periodic box may be taken from its other atom, so its force
+                                can differ in the last bits from the floor(box/rc) grid's */
 };
 
 typedef struct md_rl_params {
     int n;              /* atoms */
     float box[3];       /* periodic box edges (A) */
     float rc;           /* cutoff (A) */
-    int ncell[3];       /* cells per axis; 0 = floor(box/rc). Cell edge must be >= rc */
+    int ncell[3];       /* cells per axis; 0 = floor(box/rc) per axis (with MDRL_COARSE_CELLS,
+                           then one fewer on the axis with the most cells, ties to the shorter
+                           edge, while the average cell holds fewer than 128 atoms). Cell edge
+                           must be >= rc */
     float dt;           /* time step (fs) */
     int ntypes;         /* atom types (1..8) */
     const float *eps;   /* ntypes: LJ epsilon (kJ/mol), Lorentz-Berthelot mixing */

@@ -329,6 +329,25 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
             G.box[d] = L;
             G.inv_edge[d] = (float)((double)nc / (double)L);
         }
+        if ((p->flags & MDRL_COARSE_CELLS) && !(nsys == 1 && p->ncell[0] > 0)) {
+            // Coarser default grid while the average cell holds fewer than kMinCellAtoms
+            // atoms (one cell fewer on the axis with the most cells, ties to the shorter
+            // edge).  Sparse systems then fill a force task's 64-atom chunks and amortise
+            // its neighbour staging over more home atoms.  The pair set does not depend on
+            // the grid (cell edges stay >= rc), but which atom of a pair is the home one
+            // does, and across the periodic box that changes the rounding of the staged
+            // image, so forces are bit-exact against the oracle on the same grid.
+            for (;;) {
+                const long ncell = (long)G.nc[0] * G.nc[1] * G.nc[2];
+                if (ncell <= 1 || (double)nat[r] >= (double)kMinCellAtoms * (double)ncell) break;
+                int dm = 0;
+                for (int d = 1; d < 3; ++d)
+                    if (G.nc[d] > G.nc[dm] || (G.nc[d] == G.nc[dm] && G.box[d] / G.nc[d] < G.box[dm] / G.nc[dm]))
+                        dm = d;
+                G.nc[dm] -= 1;
+                G.inv_edge[dm] = (float)((double)G.nc[dm] / (double)G.box[dm]);
+            }
+        }
         G.atom_off = atom_off;
         G.cell_off = cell_off;
         G.n = nat[r];

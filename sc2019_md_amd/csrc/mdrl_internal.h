@@ -55,6 +55,10 @@ struct SysGeom {
     int ke_off, ke_n;       // the system's kinetic-energy chunks (kKeChunk slots each)
 };
 
+// With MDRL_COARSE_CELLS the default grid holds at least this many atoms per cell on
+// average (or one cell).
+constexpr int kMinCellAtoms = 128;
+
 // Kinetic energies are reduced in chunks of at most kKeChunk slots within one system (one
 // workgroup each, a fixed tree), then per system over its chunks.
 constexpr int kKeChunk = 2048;

@@ -27,7 +27,7 @@ ERRORS = {
 # pair flags
 PAIR_PARITY, PAIR_STRICT, PAIR_COULOMB = 1, 2, 4
 # system flags
-ACC_FP32, EVAL_ANALYTIC, EVAL_PARITY, KICK_DRIFT, NO_COULOMB = 1, 2, 4, 8, 16
+ACC_FP32, EVAL_ANALYTIC, EVAL_PARITY, KICK_DRIFT, NO_COULOMB, COARSE_CELLS = 1, 2, 4, 8, 16, 32
 # sticky kernel flags (Stats.flags)
 F_SATURATED, F_NONFINITE, F_CELL_OVERFLOW = 1, 2, 4
 TERM_14, TERM_8, TERM_3, TERM_12, TERM_6, TERM_1 = range(6)
