@@ -168,10 +168,10 @@ typedef struct md_rl_params {
     int n;              /* atoms */
     float box[3];       /* periodic box edges (A) */
     float rc;           /* cutoff (A) */
-    int ncell[3];       /* cells per axis; 0 = floor(box/rc) per axis (with MDRL_COARSE_CELLS,
-                           then one fewer on the axis with the most cells, ties to the shorter
-                           edge, while the average cell holds fewer than 128 atoms). Cell edge
-                           must be >= rc */
+    int ncell[3];       /* cells per axis; 0 = floor(box/rc) (with MDRL_COARSE_CELLS, then one
+                           fewer on the axis of those with the most cells, ties to the shorter
+                           edge, while the average cell holds fewer than 128 atoms). Single
+                           systems only; batches use 0. Cell edge must be >= rc */
     float dt;           /* time step (fs) */
     int ntypes;         /* atom types (1..8) */
     const float *eps;   /* ntypes: LJ epsilon (kJ/mol), Lorentz-Berthelot mixing */

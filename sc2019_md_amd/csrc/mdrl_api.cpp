@@ -329,7 +329,7 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
             G.box[d] = L;
             G.inv_edge[d] = (float)((double)nc / (double)L);
         }
-        if ((p->flags & MDRL_COARSE_CELLS) && !(nsys == 1 && p->ncell[0] > 0)) {
+        if (p->flags & MDRL_COARSE_CELLS) {
             // Coarser default grid while the average cell holds fewer than kMinCellAtoms
             // atoms (one cell fewer on the axis with the most cells, ties to the shorter
             // edge).  Sparse systems then fill a force task's 64-atom chunks and amortise
@@ -337,13 +337,17 @@ static int create_common(md_rl_ctx **out, const md_rl_params *p, int nsys, const
             // the grid (cell edges stay >= rc), but which atom of a pair is the home one
             // does, and across the periodic box that changes the rounding of the staged
             // image, so forces are bit-exact against the oracle on the same grid.
+            // (axes whose count a single system's params give stay as given)
+            auto free_axis = [&](int d) { return !(nsys == 1 && p->ncell[d] > 0) && G.nc[d] > 1; };
             for (;;) {
                 const long ncell = (long)G.nc[0] * G.nc[1] * G.nc[2];
-                if (ncell <= 1 || (double)nat[r] >= (double)kMinCellAtoms * (double)ncell) break;
-                int dm = 0;
-                for (int d = 1; d < 3; ++d)
-                    if (G.nc[d] > G.nc[dm] || (G.nc[d] == G.nc[dm] && G.box[d] / G.nc[d] < G.box[dm] / G.nc[dm]))
+                if ((double)nat[r] >= (double)kMinCellAtoms * (double)ncell) break;
+                int dm = -1;
+                for (int d = 0; d < 3; ++d)
+                    if (free_axis(d) && (dm < 0 || G.nc[d] > G.nc[dm] ||
+                                         (G.nc[d] == G.nc[dm] && G.box[d] / G.nc[d] < G.box[dm] / G.nc[dm])))
                         dm = d;
+                if (dm < 0) break;
                 G.nc[dm] -= 1;
                 G.inv_edge[dm] = (float)((double)G.nc[dm] / (double)G.box[dm]);
             }

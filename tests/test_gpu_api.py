@@ -11,14 +11,18 @@ from sc2019_md_amd import mdrl
 pytestmark = pytest.mark.gpu
 
 
-def coarse_grid(n, box, rc):
-    """The grid of MDRL_COARSE_CELLS, restated: floor(L / rc) per axis, then one cell fewer
-    on the axis with the most cells (ties: the shorter edge, then the lower axis) while the
-    average cell holds fewer than 128 atoms."""
-    nc = [max(1, int(math.floor(float(x) / rc))) for x in box]
-    while nc[0] * nc[1] * nc[2] > 1 and n < 128 * nc[0] * nc[1] * nc[2]:
-        dm = 0
-        for d in (1, 2):
+def coarse_grid(n, box, rc, given=(0, 0, 0)):
+    """The grid of MDRL_COARSE_CELLS, restated: floor(L / rc) per axis (or the count given),
+    then one cell fewer on the axis with the most cells among those not given and above one
+    cell (ties: the shorter edge, then the lower axis) while the average cell holds fewer
+    than 128 atoms."""
+    nc = [g if g > 0 else max(1, int(math.floor(float(x) / rc))) for g, x in zip(given, box)]
+    while n < 128 * nc[0] * nc[1] * nc[2]:
+        free = [d for d in range(3) if given[d] <= 0 and nc[d] > 1]
+        if not free:
+            break
+        dm = free[0]
+        for d in free[1:]:
             if nc[d] > nc[dm] or (nc[d] == nc[dm] and float(box[d]) / nc[d] < float(box[dm]) / nc[dm]):
                 dm = d
         nc[dm] -= 1
@@ -45,11 +49,14 @@ def test_coarse_cells_grid():
     assert list(nc) == [4, 4, 4]
     b.close()
     s = systems[0]
-    g = mdrl.System(s.n, s.box, s.rc, s.eps, s.sigma, s.mass, flags=mdrl.COARSE_CELLS, ncell=(4, 4, 4))
-    nc = (ctypes.c_int * 3)()
-    assert L.md_rl_system_size(g.handle, 0, None, None, nc) == 0
-    assert list(nc) == [4, 4, 4]
-    g.close()
+    for given in ((4, 4, 4), (4, 0, 0), (0, 0, 0)):
+        g = mdrl.System(s.n, s.box, s.rc, s.eps, s.sigma, s.mass, flags=mdrl.COARSE_CELLS, ncell=given)
+        nc = (ctypes.c_int * 3)()
+        assert L.md_rl_system_size(g.handle, 0, None, None, nc) == 0
+        assert list(nc) == coarse_grid(s.n, s.box, s.rc, given), given
+        g.close()
+    assert coarse_grid(s.n, s.box, s.rc, (4, 4, 4)) == [4, 4, 4]
+    assert coarse_grid(s.n, s.box, s.rc, (4, 0, 0)) == [4, 2, 3]
 
 
 def test_batch_geometry_queries():
