@@ -330,7 +330,7 @@ int md_rl_graph_prepare(md_rl_ctx *c, int nsteps);
 
 /* Force-kernel schedule.  atoms_per_wave: home atoms per 64-lane wavefront (16, 32 or
  * 64; below 64, 64/apw lanes share a home atom and split its neighbours); tile_depth:
- * neighbour atoms staged in LDS per pass (128 or 256, or 512 with 64 atoms per wave;
+ * neighbour atoms staged in LDS per pass (128 or 256, or 640 with 64 atoms per wave;
  * neighbour cells that fit are packed into one pass); offset_groups: workgroups per home
  * cell, splitting its 14 half-shell offsets (1, 2 or 4); persistent: a fixed grid draining
  * a largest-first task queue instead of one workgroup per task.  Default {64, 256, 1, 1}.

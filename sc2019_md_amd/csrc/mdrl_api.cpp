@@ -1371,7 +1371,7 @@ extern "C" int md_rl_autotune(md_rl_ctx *c, int reps, md_rl_schedule *best, doub
     md_rl_get_schedule(c, &cur);
     double win_ms = 1e300;
     for (int apw : {64, 32, 16})
-        for (int tile : {512, 256, 128})
+        for (int tile : {640, 256, 128})
             for (int groups : {4, 2, 1})
                 for (int pers : {0, 1}) {
                     md_rl_schedule t{apw, tile, groups, pers};

@@ -352,8 +352,8 @@ __device__ __forceinline__ int wave_prefix(int x)
 constexpr int TI = 256, NWAVE = 12, NTHR = 64 * NWAVE, NCHUNK = 4;
 // per-wave pair queue: fewer than 64 entries between steps, up to 64 more per chunk of a step
 constexpr int QCAP = 64 + 64 * NCHUNK;
-// queue entry: home index (8 bits) | neighbour index (9 bits) << 8 | type pair << 17
-constexpr unsigned kJbShift = 8, kJbMask = 0x1ffu, kTpShift = 17;
+// queue entry: home index (8 bits) | neighbour index (10 bits) << 8 | type pair << 18
+constexpr unsigned kJbShift = 8, kJbMask = 0x3ffu, kTpShift = 18;
 static_assert(TI <= 256, "queue entries hold 8-bit home indices");
 
 // One neighbour tile: offsets [o, oe) of the task's group, each a whole cell (cells that
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
     constexpr int STR = FSTR;            // floats per entry of the energy pass's window
     constexpr int NACC = EN == 1 ? 8 : 2;   // EN: E, E(rc)[, W xx xy xz yy yz zz]
     constexpr int P = 64 / APW;
-    static_assert(TJ <= 512 && TJ <= NTHR, "queue entries hold 9-bit neighbour indices");
+    static_assert(TJ <= 1024 && TJ <= NTHR, "queue entries hold 10-bit neighbour indices");
     static_assert(kBondedBlockSpan <= TI, "a bonded block accumulates in the home-tile accumulators");
     extern __shared__ float s_tab[];
     __shared__ float4 s_i[TI];
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 6))) vo
                 const float4 csh = s_cshift[cur.o];   // the tile's shift (single-cell tiles)
                 const float sx = csh.x, sy = csh.y, sz = csh.z;
 
-                // queue entry: home index + neighbour index << 8 + type pair << 17.  The scan
+                // queue entry: home index + neighbour index << 8 + type pair << 18.  The scan
                 // queues r2 <= rc2; a pair at r2 == 0 (coincident atoms, outside the
                 // reference's 0 < r2 filter) or an excluded pair is dropped here, uncounted.
                 auto eval = [&](unsigned e) -> bool {
@@ -1082,7 +1082,7 @@ bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
     if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
     if (s.excl && !fast) return false;   // exclusions: the int64 table path only
     if (sc.apw == 64 && sc.tile == 256) return true;
-    if (sc.tile == 512) return fast && sc.apw == 64;   // two or more cells per pass
+    if (sc.tile == 640) return fast && sc.apw == 64;   // two or more cells per pass
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
@@ -1118,8 +1118,8 @@ static void launch_rl(const ForceArgs &a0, dim3 grid, size_t lds, hipStream_t st
 template <bool COUL, bool EXC>
 static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, size_t lds, hipStream_t st)
 {
-    if (sc.tile == 512) {
-        launch_rl<COUL, false, false, false, 64, 512, EXC>(a, grid, lds, st);
+    if (sc.tile == 640) {
+        launch_rl<COUL, false, false, false, 64, 640, EXC>(a, grid, lds, st);
     } else if (sc.tile == 128) {
         if (sc.apw == 16) launch_rl<COUL, false, false, false, 16, 128, EXC>(a, grid, lds, st);
         else if (sc.apw == 32) launch_rl<COUL, false, false, false, 32, 128, EXC>(a, grid, lds, st);

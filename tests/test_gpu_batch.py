@@ -91,8 +91,8 @@ def test_schedules_bit_identical(which):
     ref = get()
     npairs = g.stats().pair_evals
     for apw in (16, 32, 64):
-        for tile in (128, 256, 512):
-            if tile == 512 and apw != 64:
+        for tile in (128, 256, 640):
+            if tile == 640 and apw != 64:
                 continue
             for groups in (1, 2, 4):
                 for pers in (0, 1):

@@ -64,8 +64,8 @@ def test_chain_forces_every_schedule(apoa1):
     s = mdrl.SyntheticSystem(5, 0)
     want, wpairs, _, _ = oracle_forces(s, apoa1)
     g = mdrl.System.from_synthetic(s)
-    for sc in [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 512) for gr in (1, 4) for p in (0, 1)
-               if t != 512 or a == 64]:
+    for sc in [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 640) for gr in (1, 4) for p in (0, 1)
+               if t != 640 or a == 64]:
         say("chains: schedule", sc)
         g.set_schedule(*sc)
         g.force_eval()

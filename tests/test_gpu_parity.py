@@ -431,8 +431,8 @@ def test_ragged_cells_bit_exact(apoa1, mode):
     want, npairs = rl_from_synthetic(s, apoa1, mode=omode).forces_fix()
     scheds = [(64, 256, 1, 1)]
     if mode == "fast":
-        scheds = [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 512) for gr in (1, 2, 4) for p in (0, 1)
-                  if t != 512 or a == 64]
+        scheds = [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 640) for gr in (1, 2, 4) for p in (0, 1)
+                  if t != 640 or a == 64]
     for sc in scheds:
         g.set_schedule(*sc)
         g.force_eval()
