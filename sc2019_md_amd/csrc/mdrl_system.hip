@@ -1078,11 +1078,12 @@ static void offset_groups(int ngroups, int *gstart)
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
 {
     const bool fast = !(s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC));
+    const bool ana_fix = (s.flags & MDRL_EVAL_ANALYTIC) && !(s.flags & MDRL_ACC_FP32);
     const bool groups_ok = sc.groups == 1 || sc.groups == 2 || sc.groups == 4;
     if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
     if (s.excl && !fast) return false;   // exclusions: the int64 table path only
     if (sc.apw == 64 && sc.tile == 256) return true;
-    if (sc.tile == 640) return fast && sc.apw == 64;   // two or more cells per pass
+    if (sc.tile == 640) return (fast || ana_fix) && sc.apw == 64;   // two or more cells per pass
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
@@ -1169,8 +1170,16 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     const dim3 grid(grid_n);
     const Schedule &sc = s.sched;
     if (ana) {
-        if (coul) { if (f32) launch_rl<true, true, false, true, 64, 256>(a, grid, lds, st); else launch_rl<true, false, false, true, 64, 256>(a, grid, lds, st); }
-        else { if (f32) launch_rl<false, true, false, true, 64, 256>(a, grid, lds, st); else launch_rl<false, false, false, true, 64, 256>(a, grid, lds, st); }
+        if (f32) {
+            if (coul) launch_rl<true, true, false, true, 64, 256>(a, grid, lds, st);
+            else launch_rl<false, true, false, true, 64, 256>(a, grid, lds, st);
+        } else if (sc.tile == 640) {
+            if (coul) launch_rl<true, false, false, true, 64, 640>(a, grid, lds, st);
+            else launch_rl<false, false, false, true, 64, 640>(a, grid, lds, st);
+        } else {
+            if (coul) launch_rl<true, false, false, true, 64, 256>(a, grid, lds, st);
+            else launch_rl<false, false, false, true, 64, 256>(a, grid, lds, st);
+        }
     } else if (par) {
         if (coul) { if (f32) launch_rl<true, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<true, false, true, false, 64, 256>(a, grid, lds, st); }
         else { if (f32) launch_rl<false, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<false, false, true, false, 64, 256>(a, grid, lds, st); }

@@ -87,14 +87,18 @@ def test_chain_forces_without_exclusions(apoa1, mode):
     omode = {"fast": 0, "parity": O.MODE_PARITY, "analytic": O.MODE_ANALYTIC}[mode]
     say("chains without exclusions: mode %s" % mode)
     g = mdrl.System.from_synthetic(s, flags=flags)
-    got = g.forces_fixed()
-    npairs = g.stats().pair_evals
-    g.close()
     want, wpairs, _, _ = oracle_forces(s, apoa1, mode=omode)
-    assert npairs == wpairs
-    bad = np.nonzero(np.any(got != want, axis=1))[0]
-    assert bad.size == 0, "atoms differ: %s" % bad[:10]
-    assert np.all(got.sum(axis=0) == 0)
+    scheds = {"fast": [(64, 256, 1, 1), (64, 640, 1, 1)], "parity": [(64, 256, 1, 1)],
+              "analytic": [(64, 256, 1, 1), (64, 640, 1, 1)]}[mode]
+    for sc in scheds:
+        g.set_schedule(*sc)
+        g.force_eval()
+        got = g.forces_fixed()
+        assert g.stats().pair_evals == wpairs, sc
+        bad = np.nonzero(np.any(got != want, axis=1))[0]
+        assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
+        assert np.all(got.sum(axis=0) == 0)
+    g.close()
 
 
 def test_chain_forces_fp32_atomics(apoa1):

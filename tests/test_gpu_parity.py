@@ -137,12 +137,44 @@ def test_halfshell_int64_bit_exact(apoa1, which, index, mode):
     g = gpu_system(s, flags)
     got = g.forces_fixed()
     st = g.stats()
-    g.close()
     rl = rl_from_synthetic(s, apoa1, mode=omode)
     want, npairs = rl.forces_fix()
     assert st.pair_evals == npairs
     bad = np.nonzero(np.any(got != want, axis=1))[0]
     assert bad.size == 0, "atoms differ: %s" % bad[:10]
+    if mode == "analytic":
+        # the 640-atom neighbour tile of the analytic mode, with and without the task queue
+        for sc in [(64, 640, 1, 1), (64, 640, 4, 0)]:
+            g.set_schedule(*sc)
+            g.force_eval()
+            assert g.stats().pair_evals == npairs, sc
+            bad = np.nonzero(np.any(g.forces_fixed() != want, axis=1))[0]
+            assert bad.size == 0, "schedule %s: atoms differ: %s" % (sc, bad[:10])
+    g.close()
+
+
+def test_schedules_per_mode():
+    """Which schedules each pair mode takes: the table mode every listed one, the analytic
+    mode tiles 256 and 640 at 64 atoms per wave, the parity mode and FP32 accumulation
+    only the default; the others raise and leave the schedule as it was."""
+    s = mdrl.SyntheticSystem(4, 3)
+    for flags, ok in [(0, {(a, t) for a in (16, 32, 64) for t in (128, 256)} | {(64, 640)}),
+                      (mdrl.EVAL_ANALYTIC, {(64, 256), (64, 640)}),
+                      (mdrl.EVAL_PARITY, {(64, 256)}),
+                      (mdrl.ACC_FP32, {(64, 256)}),
+                      (mdrl.EVAL_ANALYTIC | mdrl.ACC_FP32, {(64, 256)})]:
+        g = gpu_system(s, flags)
+        for a in (16, 32, 64):
+            for t in (128, 256, 512, 640):
+                before = g.schedule().as_dict()
+                if (a, t) in ok:
+                    g.set_schedule(a, t, 1, 1)
+                    assert (g.schedule().atoms_per_wave, g.schedule().tile_depth) == (a, t)
+                else:
+                    with pytest.raises(mdrl.MdrlError):
+                        g.set_schedule(a, t, 1, 1)
+                    assert g.schedule().as_dict() == before, (flags, a, t)
+        g.close()
 
 
 @pytest.mark.parametrize("which,index", [(1, 0), (4, 3), (2, 0)])
@@ -430,6 +462,8 @@ def test_ragged_cells_bit_exact(apoa1, mode):
     assert st.max_cell_count > 512
     want, npairs = rl_from_synthetic(s, apoa1, mode=omode).forces_fix()
     scheds = [(64, 256, 1, 1)]
+    if mode == "analytic":
+        scheds = [(64, t, gr, p) for t in (256, 640) for gr in (1, 4) for p in (0, 1)]
     if mode == "fast":
         scheds = [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 640) for gr in (1, 2, 4) for p in (0, 1)
                   if t != 640 or a == 64]
