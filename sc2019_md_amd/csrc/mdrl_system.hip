@@ -1836,7 +1836,7 @@ int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st, int mode)
             MDRL_HIP(hipGetLastError());
             return MDRL_OK;
         }
-        // the energy pass: every home cell with all 14 offsets, the default tile shape, cells
+        // the energy pass: every home cell with all 14 offsets, 640-atom neighbour tiles, cells
         // dealt round-robin to a fixed grid (so the sums do not depend on the force schedule)
         ForceArgs a{};
         a.pos = s.pos; a.type = s.type; a.cell_start = s.cell_start;
@@ -1860,19 +1860,19 @@ int launch_energy(DeviceSystem &s, float4 *vout, hipStream_t st, int mode)
         const dim3 g(grid_n);
         if (vir) {
             if (s.coul) {
-                if (s.excl) launch_rl<true, false, false, false, 64, 256, true, 1>(a, g, lds, st);
-                else launch_rl<true, false, false, false, 64, 256, false, 1>(a, g, lds, st);
+                if (s.excl) launch_rl<true, false, false, false, 64, 640, true, 1>(a, g, lds, st);
+                else launch_rl<true, false, false, false, 64, 640, false, 1>(a, g, lds, st);
             } else {
-                if (s.excl) launch_rl<false, false, false, false, 64, 256, true, 1>(a, g, lds, st);
-                else launch_rl<false, false, false, false, 64, 256, false, 1>(a, g, lds, st);
+                if (s.excl) launch_rl<false, false, false, false, 64, 640, true, 1>(a, g, lds, st);
+                else launch_rl<false, false, false, false, 64, 640, false, 1>(a, g, lds, st);
             }
         } else {
             if (s.coul) {
-                if (s.excl) launch_rl<true, false, false, false, 64, 256, true, 2>(a, g, lds, st);
-                else launch_rl<true, false, false, false, 64, 256, false, 2>(a, g, lds, st);
+                if (s.excl) launch_rl<true, false, false, false, 64, 640, true, 2>(a, g, lds, st);
+                else launch_rl<true, false, false, false, 64, 640, false, 2>(a, g, lds, st);
             } else {
-                if (s.excl) launch_rl<false, false, false, false, 64, 256, true, 2>(a, g, lds, st);
-                else launch_rl<false, false, false, false, 64, 256, false, 2>(a, g, lds, st);
+                if (s.excl) launch_rl<false, false, false, false, 64, 640, true, 2>(a, g, lds, st);
+                else launch_rl<false, false, false, false, 64, 640, false, 2>(a, g, lds, st);
             }
         }
         if (s.nterms > 0)
