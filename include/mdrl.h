@@ -334,8 +334,8 @@ int md_rl_graph_prepare(md_rl_ctx *c, int nsteps);
  * neighbour cells that fit are packed into one pass); offset_groups: workgroups per home
  * cell, splitting its 14 half-shell offsets (1, 2 or 4); persistent: a fixed grid draining
  * a largest-first task queue instead of one workgroup per task.  Default {64, 256, 1, 1}.
- * MDRL_ACC_FP32 and MDRL_EVAL_PARITY take only tile 256 at 64 atoms per wave, and
- * MDRL_EVAL_ANALYTIC only tiles 256 and 640 at 64 (MDRL_EINVAL otherwise).
+ * MDRL_ACC_FP32, MDRL_EVAL_PARITY and MDRL_EVAL_ANALYTIC take tiles 256 and 640 at 64
+ * atoms per wave only (MDRL_EINVAL otherwise).
  * All schedules give bit-identical int64 forces. */
 typedef struct md_rl_schedule {
     int atoms_per_wave;

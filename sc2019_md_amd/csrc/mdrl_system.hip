@@ -1078,12 +1078,10 @@ static void offset_groups(int ngroups, int *gstart)
 bool schedule_supported(const DeviceSystem &s, const Schedule &sc)
 {
     const bool fast = !(s.flags & (MDRL_ACC_FP32 | MDRL_EVAL_PARITY | MDRL_EVAL_ANALYTIC));
-    const bool ana_fix = (s.flags & MDRL_EVAL_ANALYTIC) && !(s.flags & MDRL_ACC_FP32);
     const bool groups_ok = sc.groups == 1 || sc.groups == 2 || sc.groups == 4;
     if (!groups_ok || (sc.persistent != 0 && sc.persistent != 1)) return false;
     if (s.excl && !fast) return false;   // exclusions: the int64 table path only
-    if (sc.apw == 64 && sc.tile == 256) return true;
-    if (sc.tile == 640) return (fast || ana_fix) && sc.apw == 64;   // two or more cells per pass
+    if (sc.apw == 64 && (sc.tile == 256 || sc.tile == 640)) return true;   // 640: cells packed per pass
     return fast && (sc.apw == 16 || sc.apw == 32 || sc.apw == 64) && (sc.tile == 128 || sc.tile == 256);
 }
 
@@ -1132,6 +1130,24 @@ static void launch_rl_fast(const ForceArgs &a, const Schedule &sc, dim3 grid, si
     }
 }
 
+// the analytic, parity and FP32-accumulation variants: 64 atoms per wave, tile TJ (analytic
+// takes precedence over parity, as in the pair function)
+template <int TJ>
+static void launch_rl_mode(const ForceArgs &a, bool coul, bool f32, bool par, bool ana, dim3 grid, size_t lds,
+                           hipStream_t st)
+{
+    if (ana) {
+        if (coul) { if (f32) launch_rl<true, true, false, true, 64, TJ>(a, grid, lds, st); else launch_rl<true, false, false, true, 64, TJ>(a, grid, lds, st); }
+        else { if (f32) launch_rl<false, true, false, true, 64, TJ>(a, grid, lds, st); else launch_rl<false, false, false, true, 64, TJ>(a, grid, lds, st); }
+    } else if (par) {
+        if (coul) { if (f32) launch_rl<true, true, true, false, 64, TJ>(a, grid, lds, st); else launch_rl<true, false, true, false, 64, TJ>(a, grid, lds, st); }
+        else { if (f32) launch_rl<false, true, true, false, 64, TJ>(a, grid, lds, st); else launch_rl<false, false, true, false, 64, TJ>(a, grid, lds, st); }
+    } else {
+        if (coul) launch_rl<true, true, false, false, 64, TJ>(a, grid, lds, st);
+        else launch_rl<false, true, false, false, 64, TJ>(a, grid, lds, st);
+    }
+}
+
 int launch_force(DeviceSystem &s, hipStream_t st)
 {
     if (s.n == 0) return MDRL_OK;
@@ -1169,23 +1185,9 @@ int launch_force(DeviceSystem &s, hipStream_t st)
     a.progress_prio = ntot <= 2 * grid_n;
     const dim3 grid(grid_n);
     const Schedule &sc = s.sched;
-    if (ana) {
-        if (f32) {
-            if (coul) launch_rl<true, true, false, true, 64, 256>(a, grid, lds, st);
-            else launch_rl<false, true, false, true, 64, 256>(a, grid, lds, st);
-        } else if (sc.tile == 640) {
-            if (coul) launch_rl<true, false, false, true, 64, 640>(a, grid, lds, st);
-            else launch_rl<false, false, false, true, 64, 640>(a, grid, lds, st);
-        } else {
-            if (coul) launch_rl<true, false, false, true, 64, 256>(a, grid, lds, st);
-            else launch_rl<false, false, false, true, 64, 256>(a, grid, lds, st);
-        }
-    } else if (par) {
-        if (coul) { if (f32) launch_rl<true, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<true, false, true, false, 64, 256>(a, grid, lds, st); }
-        else { if (f32) launch_rl<false, true, true, false, 64, 256>(a, grid, lds, st); else launch_rl<false, false, true, false, 64, 256>(a, grid, lds, st); }
-    } else if (f32) {
-        if (coul) launch_rl<true, true, false, false, 64, 256>(a, grid, lds, st);
-        else launch_rl<false, true, false, false, 64, 256>(a, grid, lds, st);
+    if (ana || par || f32) {
+        if (sc.tile == 640) launch_rl_mode<640>(a, coul, f32, par, ana, grid, lds, st);
+        else launch_rl_mode<256>(a, coul, f32, par, ana, grid, lds, st);
     } else if (s.excl) {
         if (coul) launch_rl_fast<true, true>(a, sc, grid, lds, st);
         else launch_rl_fast<false, true>(a, sc, grid, lds, st);

@@ -88,7 +88,7 @@ def test_chain_forces_without_exclusions(apoa1, mode):
     say("chains without exclusions: mode %s" % mode)
     g = mdrl.System.from_synthetic(s, flags=flags)
     want, wpairs, _, _ = oracle_forces(s, apoa1, mode=omode)
-    scheds = {"fast": [(64, 256, 1, 1), (64, 640, 1, 1)], "parity": [(64, 256, 1, 1)],
+    scheds = {"fast": [(64, 256, 1, 1), (64, 640, 1, 1)], "parity": [(64, 256, 1, 1), (64, 640, 1, 1)],
               "analytic": [(64, 256, 1, 1), (64, 640, 1, 1)]}[mode]
     for sc in scheds:
         g.set_schedule(*sc)

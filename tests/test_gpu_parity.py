@@ -142,8 +142,8 @@ def test_halfshell_int64_bit_exact(apoa1, which, index, mode):
     assert st.pair_evals == npairs
     bad = np.nonzero(np.any(got != want, axis=1))[0]
     assert bad.size == 0, "atoms differ: %s" % bad[:10]
-    if mode == "analytic":
-        # the 640-atom neighbour tile of the analytic mode, with and without the task queue
+    if mode != "fast":
+        # the 640-atom neighbour tile of the other modes, with and without the task queue
         for sc in [(64, 640, 1, 1), (64, 640, 4, 0)]:
             g.set_schedule(*sc)
             g.force_eval()
@@ -155,14 +155,13 @@ def test_halfshell_int64_bit_exact(apoa1, which, index, mode):
 
 def test_schedules_per_mode():
     """Which schedules each pair mode takes: the table mode every listed one, the analytic
-    mode tiles 256 and 640 at 64 atoms per wave, the parity mode and FP32 accumulation
-    only the default; the others raise and leave the schedule as it was."""
+    and parity modes and FP32 accumulation tiles 256 and 640 at 64 atoms per wave; the
+    others raise and leave the schedule as it was."""
     s = mdrl.SyntheticSystem(4, 3)
+    other = {(64, 256), (64, 640)}
     for flags, ok in [(0, {(a, t) for a in (16, 32, 64) for t in (128, 256)} | {(64, 640)}),
-                      (mdrl.EVAL_ANALYTIC, {(64, 256), (64, 640)}),
-                      (mdrl.EVAL_PARITY, {(64, 256)}),
-                      (mdrl.ACC_FP32, {(64, 256)}),
-                      (mdrl.EVAL_ANALYTIC | mdrl.ACC_FP32, {(64, 256)})]:
+                      (mdrl.EVAL_ANALYTIC, other), (mdrl.EVAL_PARITY, other), (mdrl.ACC_FP32, other),
+                      (mdrl.EVAL_ANALYTIC | mdrl.ACC_FP32, other), (mdrl.EVAL_PARITY | mdrl.ACC_FP32, other)]:
         g = gpu_system(s, flags)
         for a in (16, 32, 64):
             for t in (128, 256, 512, 640):
@@ -182,8 +181,13 @@ def test_halfshell_fp32_atomics_close(which, index):
     s = mdrl.SyntheticSystem(which, index)
     say("halfshell fp32: workload %d replica %d" % (which, index))
     a = gpu_system(s).forces_fixed().astype(np.float64)
-    b = gpu_system(s, mdrl.ACC_FP32).forces_fixed().astype(np.float64)
-    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-5
+    g = gpu_system(s, mdrl.ACC_FP32)
+    for tile in (256, 640):
+        g.set_schedule(64, tile, 1, 1)
+        g.force_eval()
+        b = g.forces_fixed().astype(np.float64)
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-5, tile
+    g.close()
 
 
 def test_small_boxes_all_images(apoa1):
@@ -462,7 +466,7 @@ def test_ragged_cells_bit_exact(apoa1, mode):
     assert st.max_cell_count > 512
     want, npairs = rl_from_synthetic(s, apoa1, mode=omode).forces_fix()
     scheds = [(64, 256, 1, 1)]
-    if mode == "analytic":
+    if mode != "fast":
         scheds = [(64, t, gr, p) for t in (256, 640) for gr in (1, 4) for p in (0, 1)]
     if mode == "fast":
         scheds = [(a, t, gr, p) for a in (16, 32, 64) for t in (128, 256, 640) for gr in (1, 2, 4) for p in (0, 1)
