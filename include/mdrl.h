@@ -333,7 +333,7 @@ int md_rl_graph_prepare(md_rl_ctx *c, int nsteps);
  * neighbour atoms staged in LDS per pass (128 or 256, or 640 with 64 atoms per wave;
  * neighbour cells that fit are packed into one pass); offset_groups: workgroups per home
  * cell, splitting its 14 half-shell offsets (1, 2 or 4); persistent: a fixed grid draining
- * a largest-first task queue instead of one workgroup per task.  Default {64, 256, 1, 1}.
+ * a largest-first task queue instead of one workgroup per task.  Default {64, 640, 1, 1}.
  * MDRL_ACC_FP32, MDRL_EVAL_PARITY and MDRL_EVAL_ANALYTIC take tiles 256 and 640 at 64
  * atoms per wave only (MDRL_EINVAL otherwise).
  * All schedules give bit-identical int64 forces. */

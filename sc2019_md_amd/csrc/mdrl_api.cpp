@@ -652,7 +652,7 @@ static int upload_topology(md_rl_ctx *c)
         if (!s.excl && (rc = dalloc(&s.excl, (size_t)s.n))) return rc;
         if ((e = hipMemcpyAsync(s.excl, m.data(), sizeof(uint32_t) * m.size(), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
             return hip_fail(e, "hipMemcpyAsync");
-        if (!schedule_supported(s, s.sched)) { s.sched.apw = 64; s.sched.tile = 256; }
+        if (!schedule_supported(s, s.sched)) { s.sched.apw = 64; s.sched.tile = 640; }
     } else if (s.excl) {
         (void)hipFree(s.excl);
         s.excl = nullptr;

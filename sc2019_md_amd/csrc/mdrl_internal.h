@@ -67,7 +67,7 @@ constexpr int kKeChunk = 2048;
 // half-shell offset groups per workgroup, persistent work queue.
 struct Schedule {
     int apw = 64;
-    int tile = 256;
+    int tile = 640;
     int groups = 1;
     int persistent = 1;
 };

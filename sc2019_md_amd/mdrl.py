@@ -495,7 +495,7 @@ class System:
         """Capture (without running) the graphs that step(nsteps) would replay."""
         _check(lib().md_rl_graph_prepare(self.handle, nsteps))
 
-    def set_schedule(self, apw=64, tile=256, groups=1, persistent=1):
+    def set_schedule(self, apw=64, tile=640, groups=1, persistent=1):
         sc = Schedule(apw, tile, groups, persistent)
         _check(lib().md_rl_set_schedule(self.handle, ctypes.byref(sc)))
 
